@@ -1,0 +1,176 @@
+"""Benchmark: Vecchia loglik/grad/Fisher evaluations per second at n=50k, d=8, m=30.
+
+Workload (one step = one objective evaluation: loglik + gradient (P=10) +
+Fisher (10x10) + GLS beta (q=9)) on the n=50,000, d=8 synthetic test
+function with a linear trend, nu=2.5, m_est=30, at the default starting
+parameters of the fit and on the structure the fit builds first (maximin
+order + ordered 30-NN in the scaled space).  Inputs are resident on the
+device before the timed region.
+
+Prints one JSON line.  Extra keys: the fit time of the same problem, the
+rebuild (maximin + kNN) times, the dominant kernel's roofline, and a CPU
+baseline (the NumPy oracle on a bounded sample of the same blocks).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 vector / matrix
+A_COEF = np.array([1.0, 0.8, 0.6, 0.4, 0.2, 0.05, 0.0, 0.0])
+ELL = np.array([0.5, 0.7, 1.0, 1.5, 2.0, 3.0, 1.0, 1.0])
+
+
+def c2_data(n=50_000, d=8, seed=1002):
+    """X ~ U[0,1]^8; y = sum_l a_l sin(2 pi x_l / ell_l) + 0.1 x1 x2 + N(0, 1e-4)."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, d))
+    y = (A_COEF * np.sin(2 * np.pi * X / ELL)).sum(1) + 0.1 * X[:, 0] * X[:, 1]
+    y = y + rng.normal(0.0, 0.01, n)
+    return X, y
+
+
+def block_flops(K, d, P, q):
+    """Algorithmic FP64 flops of one block evaluation (FMA = 2): pairwise
+    distance/Matern/g, dS.a for the d ranges, Cholesky, a and W solves, Fisher,
+    GLS, z and gradient terms."""
+    T = K * (K + 1) / 2
+    return (T * (3 * d + 10 + 5 * d) + K ** 3 / 3 + (1 + P) * K ** 2 + P * (P + 1) * K
+            + 2 * K * (1 + q) + q * (q + 1) + 2 * q + K ** 2 + 2 * K * q + 2 * P * K)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--n", type=int, default=50_000)
+    ap.add_argument("--no-fit", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    from pyscaledvecchia_amd import ScaledVecchiaGP, _lib
+    from pyscaledvecchia_amd.likelihood import VecchiaStructure
+    from pyscaledvecchia_amd.ordering import _nn_groups, find_ordered_nn, maximin_order
+
+    n, d, m = args.n, 8, 30
+    X, y = c2_data(n, d)
+    # the fit's own preprocessing (gp.py:149-177)
+    lo, span = X.min(0), X.max(0) - X.min(0)
+    Xe = (X - lo) / span
+    ye = (y - y.mean()) / y.std()
+    Ze = np.column_stack([np.ones(n), Xe])
+    theta = np.concatenate([[math.log(ye.var())], np.full(d, math.log(0.2 * math.sqrt(d))),
+                            [math.log(0.1)]])
+    lvt = math.log(ye.var())
+    ctx = _lib.context()
+
+    # rebuild (timed separately, not part of the step)
+    Xs = Xe / np.exp(theta[1:1 + d])
+    maximin_order(Xs[:2000])
+    t0 = time.perf_counter()
+    order = maximin_order(Xs)
+    t_mm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nn = find_ordered_nn(Xs[order], m)
+    t_knn = time.perf_counter() - t0
+    groups = _nn_groups(nn, m)
+    st = VecchiaStructure(Xe[order], ye[order], Ze[order], groups, ctx=ctx)
+    P, q = d + 2, d + 1
+
+    for _ in range(args.warmup):
+        st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=args.precision)
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll, g, F, beta = st(theta, 2.5, var_penalty=1.0, log_var_target=lvt,
+                            precision=args.precision)
+    t_steps = time.perf_counter() - t0     # every call synchronises on return
+    ctx.timing(False)
+    k_ms, k_cnt = ctx.timing_read("vecchia_pass1")
+    ms_step = 1e3 * t_steps / args.steps
+    ks = np.minimum(np.arange(n), m) + 1
+    flops = float(sum(block_flops(int(K), d, P, q) for K in ks[:m + 1]) +
+                  (n - m - 1) * block_flops(m + 1, d, P, q))
+    k_avg_s = 1e-3 * k_ms / max(k_cnt, 1)
+    achieved = flops / k_avg_s / 1e12
+
+    out = {
+        "metric": "fit s + Vecchia loglik/grad/Fisher evals/s, n=50k d=8 m=30; FP32-mix vs FP64",
+        "value": args.steps / t_steps,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (sin-sum test function, seeded)",
+        "config": {"workload": f"C2 objective eval n={n} d={d} m_est={m} nu=2.5 trend=linear",
+                   "global_batch": n, "seq_len": m + 1, "parallelism": "single-gpu",
+                   "precision": args.precision},
+        "roofline": {"bound": "mfma", "kernel": "ll_pass1", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel_avg_ms": 1e3 * k_avg_s, "flops_per_launch": flops},
+        "loglik": ll,
+        "rebuild_s": {"maximin": t_mm, "ordered_knn": t_knn},
+    }
+    # full fit of the same problem (host control loop + GPU hot path)
+    if not args.no_fit:
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        gp = ScaledVecchiaGP(m_est=m, m_pred=140, nu=2.5, trend="linear", n_est=n,
+                             precision=args.precision).fit(X, y)
+        t_fit = time.perf_counter() - t0
+        ctx.timing(False)
+        out["fit_s"] = t_fit
+        out["fit"] = {"evals": gp.n_evals_, "rebuilds": gp.n_rebuilds_,
+                      "loglik": gp.loglik_, "b": gp.b_,
+                      "maximin_s": ctx.timing_read("maximin")[0] / 1e3,
+                      "knn_s": ctx.timing_read("knn")[0] / 1e3,
+                      "pass1_s": ctx.timing_read("vecchia_pass1")[0] / 1e3,
+                      "predict_s": ctx.timing_read("predict_blocks")[0] / 1e3}
+    # CPU baseline: the NumPy oracle on a bounded prefix of the same blocks
+    if not args.no_cpu:
+        from oracle import vecchia_np as O
+        nsub = 3000
+        sub_groups = _nn_groups(nn[:nsub], m)
+        Xo, yo, Zo = Xe[order], ye[order], Ze[order]
+        O.vecchia_profile_loglik(theta, Xo, yo, Zo, sub_groups, 2.5)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            O.vecchia_profile_loglik(theta, Xo, yo, Zo, sub_groups, 2.5, var_penalty=1.0,
+                                     log_var_target=lvt)
+            reps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        per_block = (time.perf_counter() - t0) / reps / nsub
+        out["cpu_baseline"] = {"value": 1.0 / (per_block * n), "unit": "evals/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"NumPy oracle, first {nsub} blocks of the n={n} "
+                                         f"structure x{reps} evals, scaled to n={n}"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/*
+ * svh.h -- C ABI of libsvh.so, the gfx950 (MI355X) scaled-Vecchia hot path.
+ *
+ * Plain pointers and sizes only.  Every host array is caller-owned,
+ * C-contiguous, float64 / int64 / int32 as declared; no host pointer is kept
+ * past a call.  Device buffers belong to the context and persist across calls
+ * (setup once, evaluate many times).  Every call is synchronous on return.
+ * A context is not thread-safe: use one per host thread.
+ *
+ * The reference (jdtuck/pyscaledvecchia) is a Python package with no FFI; each
+ * entry point below replaces the Python function cited beside it, and the
+ * Python package pyscaledvecchia_amd binds them with ctypes.
+ */
+#ifndef SVH_H
+#define SVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SVH_OK 0
+#define SVH_ERR_CHOLESKY 1 /* block Cholesky failed after the jitter ladder  */
+#define SVH_ERR_ARG 2      /* bad argument (shape, size, value)              */
+#define SVH_ERR_HIP 3      /* HIP runtime error                              */
+
+/* precision modes of svh_vecchia_eval / svh_predict */
+#define SVH_FP64 0  /* every stage in float64                                  */
+#define SVH_MIXED 1 /* float32 derivative products / W solves / Fisher;
+                       float64 covariance, Cholesky, log-det, GLS              */
+
+typedef struct svh_ctx svh_ctx;
+
+/* number of visible HIP devices (0 when none) */
+int svh_device_count(void);
+/* library version string */
+const char* svh_version(void);
+
+/* Create a context on HIP device `device` (one stream, default priority). */
+int svh_create(int device, svh_ctx** out);
+void svh_destroy(svh_ctx* ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char* svh_last_error(const svh_ctx* ctx);
+
+/* ---------------------------------------------------------------- (a) --- */
+
+/* Exact maximin order of the n rows of X (n x d, scaled coordinates),
+ * starting at row `first`.  Ties take the lowest index.  Writes order[n].
+ * Replaces ordering.py:27-73 (_maximin_core / maximin_order; the caller
+ * computes `first` as ordering.py:72 does). */
+int svh_maximin(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int64_t first,
+                int64_t* order);
+
+/* Ordered nearest neighbours: row i of nn (n x (m+1)) gets i in column 0 and
+ * the min(i, m) nearest j < i (by squared distance, then index; nearest first)
+ * in columns 1..; -1 elsewhere.  Rows below `start` get only column 0.
+ * Replaces ordering.py:76-116 (find_ordered_nn). */
+int svh_ordered_knn(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t m,
+                    int64_t start, int64_t* nn);
+
+/* k nearest rows of Xref (nref x d) for each row of Xq (nq x d), sorted by
+ * (squared distance, index).  Writes nb (nq x k).
+ * Replaces the cKDTree query of gp.py:253-254. */
+int svh_knn_query(svh_ctx* ctx, const double* Xref, int64_t nref, const double* Xq,
+                  int64_t nq, int32_t d, int32_t k, int64_t* nb);
+
+/* ---------------------------------------------------------------- (b) --- */
+
+/* Upload one Vecchia structure (inputs already in Vecchia order).
+ *   Xo (n x d), yo (n), Zo (n x q, may be NULL when q == 0)
+ *   nblk blocks; block b conditions row rows[b] on nbr[b*mmax .. +kcnt[b]]
+ *   (its neighbours; self is placed last).  kcnt[b] <= mmax <= 127.
+ *   chunk_grad[b] / chunk_val[b]: the jitter-chunk id of block b when the
+ *   gradient is / is not requested (_covariance.py:255-259 slicing of each
+ *   conditioning-size group, likelihood.py:111-114).
+ * Replaces the per-rebuild state of gp.py:182-191 + ordering.py:119-127. */
+int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const double* Zo,
+                      int64_t n, int32_t d, int32_t q, int64_t nblk, const int64_t* rows,
+                      const int64_t* nbr, const int32_t* kcnt, int32_t mmax,
+                      const int32_t* chunk_grad, const int32_t* chunk_val, int32_t nchunk_grad,
+                      int32_t nchunk_val);
+
+/* One profile Vecchia log-likelihood evaluation on the uploaded structure.
+ *   theta (P = d+2): log sigma^2, log lambda_1..d, log tau.  nu in {0.5,1.5,2.5}.
+ *   Outputs: ll, grad[P], fisher[P*P] (row-major), beta[q]; grad/fisher are
+ *   untouched when need_grad == 0.  *jitter_power_max receives the largest
+ *   jitter power used by any chunk (0 = none, else -12..-4).
+ * Replaces likelihood.py:66-175 (vecchia_profile_loglik) for fixed nu. */
+int svh_vecchia_eval(svh_ctx* ctx, const double* theta, int32_t P, double nu,
+                     int32_t precision, int32_t need_grad, double var_penalty,
+                     double log_var_target, double* ll, double* grad, double* fisher,
+                     double* beta, int32_t* jitter_power_max);
+
+/* ---------------------------------------------------------------- (c) --- */
+
+/* Marginal prediction (gp.py:242-277, _predict_scaled): for each of the nq
+ * rows of Xq, its m nearest training rows in the scaled space x/lambda
+ * (sorted), the block [neighbours..., target] with a noise-free target
+ * (noise_free != 0), and
+ *   mean[i] = -a[0:m].resid[nb] / a[m],   var[i] = 1 / a[m]^2
+ * where a = L^-T e_K.  Xtr (ntr x d) and Xq are standardised inputs, resid
+ * (ntr) the centred responses, theta as in svh_vecchia_eval.
+ * nb_out (nq x m) receives the neighbour indices when not NULL. */
+int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t ntr,
+                const double* Xq, int64_t nq, int32_t d, int32_t m, const double* theta,
+                double nu, int32_t precision, int32_t noise_free, double* mean,
+                double* var, int64_t* nb_out);
+
+/* Columns of the sparse inverse Cholesky factor for arbitrary blocks
+ * (gp.py:332-348, _joint_factor): block b is rows blocks[b*K .. b*K+K-1] of
+ * Xall (self last), nugget mask mask[b*K + j] (NULL = all ones); writes
+ * a[b*K + j] = (L^-T e_K)_j. */
+int svh_block_columns(svh_ctx* ctx, const double* Xall, int64_t nall, int32_t d,
+                      const int64_t* blocks, const double* mask, int64_t nblk, int32_t K,
+                      const double* theta, double nu, int32_t precision, double* a);
+
+/* ------------------------------------------------------- instrumentation --- */
+/* No counterpart in the reference (which has no timers).  When enabled, the
+ * library brackets each launch of its dominant kernels with HIP events on the
+ * context stream and accumulates their durations under a kernel name
+ * ("vecchia_pass1", "predict_blocks", "knn", "maximin"; "maximin_rounds"
+ * reads the number of selection rounds of the last maximin call as `count`).
+ * svh_timing_read returns the accumulated milliseconds and launch count. */
+int svh_timing_enable(svh_ctx* ctx, int32_t enable);
+int svh_timing_read(svh_ctx* ctx, const char* name, double* total_ms, int64_t* count);
+int svh_timing_reset(svh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVH_H */

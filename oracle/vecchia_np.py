@@ -89,12 +89,13 @@ def maximin_first(X):
     return int(np.argmin(((X - X.mean(0)) ** 2).sum(1)))
 
 
-def maximin_order(X, first=None):
+def maximin_order(X, first=None, steps=None):
     """Exact maximin order, serial O(n^2 d) (ordering.py:27-73).
 
     Step t picks the unselected point with the largest distance to the
     selected set (lowest index among ties, ordering.py:43-47) and lowers every
     unselected distance that the new point beats strictly (ordering.py:48-57).
+    `steps` (test aid) stops after that many points and returns the prefix.
     """
     X = np.ascontiguousarray(X, dtype=np.float64)
     n = X.shape[0]
@@ -106,14 +107,15 @@ def maximin_order(X, first=None):
         return order[:0]
     dist = np.sqrt(sqdist_to(X, X[first]))
     dist[first] = -1.0
-    for t in range(1, n):
+    n_out = n if steps is None else min(n, steps)
+    for t in range(1, n_out):
         j = int(np.argmax(dist))             # first maximum = lowest index
         order[t] = j
         dj = np.sqrt(sqdist_to(X, X[j]))
         upd = (dist >= 0.0) & (dj < dist)
         dist[upd] = dj[upd]
         dist[j] = -1.0
-    return order
+    return order[:n_out]
 
 
 def find_ordered_nn(X, m, start=0):

@@ -1,0 +1,239 @@
+"""ctypes binding of libsvh.so (the gfx950 HIP library, include/svh.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is
+visible, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvh.so")
+
+SVH_OK, SVH_ERR_CHOLESKY, SVH_ERR_ARG, SVH_ERR_HIP = 0, 1, 2, 3
+PRECISION = {"fp64": 0, "mixed": 1}
+
+_lib = None
+_lock = threading.Lock()
+_ctx = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `make -C pyscaledvecchia_amd/csrc` "
+            "or __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    L.svh_device_count.restype = ctypes.c_int
+    L.svh_version.restype = ctypes.c_char_p
+    L.svh_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.svh_destroy.argtypes = [_vp]
+    L.svh_destroy.restype = None
+    L.svh_last_error.argtypes = [_vp]
+    L.svh_last_error.restype = ctypes.c_char_p
+    L.svh_maximin.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, _i64p]
+    L.svh_ordered_knn.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int64, _i64p]
+    L.svh_knn_query.argtypes = [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32,
+                                ctypes.c_int32, _i64p]
+    L.svh_vecchia_setup.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_int64, _i64p, _i64p, _i32p,
+                                    ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, ctypes.c_int32]
+    L.svh_vecchia_eval.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                   _dp, _dp, _i32p]
+    L.svh_predict.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32,
+                              ctypes.c_int32, _dp, ctypes.c_double, ctypes.c_int32,
+                              ctypes.c_int32, _dp, _dp, _i64p]
+    L.svh_block_columns.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, _i64p, _dp,
+                                    ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_double,
+                                    ctypes.c_int32, _dp]
+    L.svh_timing_enable.argtypes = [_vp, ctypes.c_int32]
+    L.svh_timing_read.argtypes = [_vp, ctypes.c_char_p, _dp, _i64p]
+    L.svh_timing_reset.argtypes = [_vp]
+    for name in ("svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
+                 "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh_last_error",
+            "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
+            "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_timing_enable",
+            "svh_timing_read", "svh_timing_reset")
+
+
+def device_count() -> int:
+    return int(_load().svh_device_count())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _finite(a, what):
+    a = _f64(a)
+    if not np.all(np.isfinite(a)):
+        raise ValueError(f"{what} contains NaN or infinite values")
+    return a
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """One device context (one HIP stream).  Not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        L = _load()
+        h = _vp()
+        rc = L.svh_create(int(device), ctypes.byref(h))
+        if rc != SVH_OK:
+            raise RuntimeError(f"svh_create(device={device}) failed with code {rc} "
+                               "(no HIP device?)")
+        self._h = h
+        self._L = L
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.svh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc == SVH_OK:
+            return
+        msg = self._L.svh_last_error(self._h).decode()
+        if rc == SVH_ERR_CHOLESKY:
+            raise np.linalg.LinAlgError(msg)
+        if rc == SVH_ERR_ARG:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+    # ---------------------------------------------------------------- timers
+    def timing(self, enable=True):
+        self._check(self._L.svh_timing_enable(self._h, int(bool(enable))))
+
+    def timing_reset(self):
+        self._check(self._L.svh_timing_reset(self._h))
+
+    def timing_read(self, name):
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_int64(0)
+        self._check(self._L.svh_timing_read(self._h, name.encode(), ctypes.byref(ms),
+                                            ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    # ---------------------------------------------------------------- (a)
+    def maximin(self, X, first):
+        X = _finite(X, "X")
+        n, d = X.shape
+        out = np.empty(n, dtype=np.int64)
+        self._check(self._L.svh_maximin(self._h, _ptr(X), n, d, int(first), _ptr(out, _i64p)))
+        return out
+
+    def ordered_knn(self, X, m, start=0):
+        X = _finite(X, "X")
+        n, d = X.shape
+        nn = np.empty((n, m + 1), dtype=np.int64)
+        self._check(self._L.svh_ordered_knn(self._h, _ptr(X), n, d, int(m), int(start),
+                                            _ptr(nn, _i64p)))
+        return nn
+
+    def knn_query(self, Xref, Xq, k):
+        Xref, Xq = _finite(Xref, "Xref"), _finite(Xq, "Xq")
+        nb = np.empty((Xq.shape[0], k), dtype=np.int64)
+        self._check(self._L.svh_knn_query(self._h, _ptr(Xref), Xref.shape[0], _ptr(Xq),
+                                          Xq.shape[0], Xref.shape[1], int(k), _ptr(nb, _i64p)))
+        return nb
+
+    # ---------------------------------------------------------------- (b)
+    def vecchia_setup(self, Xo, yo, Zo, rows, nbr, kcnt, mmax, chunk_grad, chunk_val,
+                      nchunk_grad, nchunk_val):
+        Xo, yo = _finite(Xo, "Xo"), _finite(yo, "yo")
+        q = 0 if Zo is None else Zo.shape[1]
+        Zo = None if q == 0 else _f64(Zo)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        nbr = np.ascontiguousarray(nbr, dtype=np.int64)
+        kcnt = np.ascontiguousarray(kcnt, dtype=np.int32)
+        cg = np.ascontiguousarray(chunk_grad, dtype=np.int32)
+        cv = np.ascontiguousarray(chunk_val, dtype=np.int32)
+        self._check(self._L.svh_vecchia_setup(
+            self._h, _ptr(Xo), _ptr(yo), _ptr(Zo), Xo.shape[0], Xo.shape[1], q, rows.shape[0],
+            _ptr(rows, _i64p), _ptr(nbr, _i64p), _ptr(kcnt, _i32p), int(mmax), _ptr(cg, _i32p),
+            _ptr(cv, _i32p), int(nchunk_grad), int(nchunk_val)))
+        self._q = q
+
+    def vecchia_eval(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
+                     precision="fp64"):
+        theta = _f64(theta)
+        P = theta.shape[0]
+        ll = np.zeros(1)
+        grad = np.zeros(P)
+        fish = np.zeros((P, P))
+        beta = np.zeros(self._q)
+        jp = ctypes.c_int32(0)
+        self._check(self._L.svh_vecchia_eval(
+            self._h, _ptr(theta), P, float(nu), PRECISION[precision], int(bool(need_grad)),
+            float(var_penalty), float(log_var_target), _ptr(ll), _ptr(grad), _ptr(fish),
+            _ptr(beta) if self._q else None, ctypes.byref(jp)))
+        self.last_jitter_power = jp.value
+        if not need_grad:
+            return float(ll[0]), None, None, beta
+        return float(ll[0]), grad, fish, beta
+
+    # ---------------------------------------------------------------- (c)
+    def predict(self, Xtr, resid, Xq, m, theta, nu, noise_free=True, precision="fp64",
+                return_nb=False):
+        Xtr, resid, Xq = _finite(Xtr, "X"), _finite(resid, "resid"), _finite(Xq, "Xstar")
+        theta = _finite(theta, "theta")
+        nq = Xq.shape[0]
+        mean = np.empty(nq)
+        var = np.empty(nq)
+        nb = np.empty((nq, m), dtype=np.int64) if return_nb else None
+        self._check(self._L.svh_predict(
+            self._h, _ptr(Xtr), _ptr(resid), Xtr.shape[0], _ptr(Xq), nq, Xtr.shape[1], int(m),
+            _ptr(theta), float(nu), PRECISION[precision], int(bool(noise_free)), _ptr(mean),
+            _ptr(var), _ptr(nb, _i64p)))
+        return (mean, var, nb) if return_nb else (mean, var)
+
+    def block_columns(self, Xall, blocks, mask, theta, nu, precision="fp64"):
+        Xall, theta = _finite(Xall, "X"), _finite(theta, "theta")
+        blocks = np.ascontiguousarray(blocks, dtype=np.int64)
+        mask = None if mask is None else _f64(mask)
+        nblk, K = blocks.shape
+        a = np.empty((nblk, K))
+        self._check(self._L.svh_block_columns(
+            self._h, _ptr(Xall), Xall.shape[0], Xall.shape[1], _ptr(blocks, _i64p), _ptr(mask),
+            nblk, K, _ptr(theta), float(nu), PRECISION[precision], _ptr(a)))
+        return a
+
+
+def context(device: int = 0) -> Context:
+    """Process-wide default context on `device` (created on first use)."""
+    global _ctx
+    with _lock:
+        if _ctx is None or _ctx.device != device:
+            _ctx = Context(device)
+        return _ctx

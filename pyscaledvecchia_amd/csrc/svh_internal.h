@@ -1,0 +1,177 @@
+// Internal declarations shared by the translation units of libsvh.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/svh.h"
+
+// ---------------------------------------------------------------- errors
+struct SvhError {
+  int code;
+  std::string msg;
+};
+
+#define SVH_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw SvhError{SVH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define SVH_REQUIRE(cond, text)                                  \
+  do {                                                           \
+    if (!(cond)) throw SvhError{SVH_ERR_ARG, std::string(text)}; \
+  } while (0)
+
+// ---------------------------------------------------------------- buffers
+// Named device buffers that only grow: repeated calls of one size allocate
+// once.  Pointers stay valid until the same name is requested larger.
+struct DevArena {
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::unordered_map<std::string, Buf> bufs;
+  void* get(const std::string& name, size_t bytes) {
+    Buf& b = bufs[name];
+    if (bytes == 0) bytes = 1;
+    if (b.bytes < bytes) {
+      if (b.p) SVH_HIP(hipFree(b.p));
+      b.p = nullptr;
+      SVH_HIP(hipMalloc(&b.p, bytes));
+      b.bytes = bytes;
+    }
+    return b.p;
+  }
+  template <class T>
+  T* get_t(const std::string& name, size_t count) {
+    return static_cast<T*>(get(name, count * sizeof(T)));
+  }
+  void release() {
+    for (auto& kv : bufs)
+      if (kv.second.p) (void)hipFree(kv.second.p);
+    bufs.clear();
+  }
+};
+
+// Pinned host staging for small results.
+struct HostStage {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t want) {
+    if (bytes < want) {
+      if (p) SVH_HIP(hipHostFree(p));
+      p = nullptr;
+      SVH_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+      bytes = want;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// State uploaded by svh_vecchia_setup.
+struct VecchiaState {
+  bool ready = false;
+  int64_t n = 0, nblk = 0;
+  int d = 0, q = 0, mmax = 0, kp = 0;
+  int nchunk_grad = 0, nchunk_val = 0;
+  // device pointers live in the arena under "v.*" names
+};
+
+// Kernel timers: HIP events on the context stream around named launches.
+struct KernelTimers {
+  bool enabled = false;
+  struct Acc {
+    double ms = 0.0;
+    int64_t count = 0;
+  };
+  std::unordered_map<std::string, Acc> acc;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    SVH_HIP(hipEventCreate(&e));
+    return e;
+  }
+  // returns the start event to record before the launch (nullptr if disabled)
+  void begin(hipStream_t s, const char* name) {
+    if (!enabled) return;
+    Pending p{name, take(), take()};
+    SVH_HIP(hipEventRecord(p.a, s));
+    pending.push_back(p);
+  }
+  void end(hipStream_t s) {
+    if (!enabled || pending.empty()) return;
+    SVH_HIP(hipEventRecord(pending.back().b, s));
+  }
+  // call after a stream synchronisation
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      SVH_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      Acc& a = acc[p.name];
+      a.ms += ms;
+      a.count += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void release() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pending.clear();
+    pool.clear();
+  }
+};
+
+struct svh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  DevArena arena;
+  HostStage stage;
+  VecchiaState vs;
+  KernelTimers timers;
+  int mm_rounds = 0;  // rounds of the last maximin call
+  std::string last_error;
+};
+
+// ---------------------------------------------------------------- launchers
+// ordering.hip
+void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n, int d, int64_t first,
+                    int64_t* dorder);
+void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m,
+                        int64_t start, int64_t* dnn);
+void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const double* dXqT,
+                      int64_t nq, int d, int k, int64_t* dnb, int nb_stride);
+// loglik.hip
+void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, int precision,
+                         int need_grad, double var_penalty, double log_var_target,
+                         double* ll, double* grad, double* fisher, double* beta,
+                         int* jitter_power_max);
+// Jitter factors 10^p, p = -12..-4 (index 1..9; index 0 = no jitter).
+extern const double kJitterPow10[10];

@@ -1,0 +1,89 @@
+"""Profile Vecchia log-likelihood, gradient and Fisher information on the GPU.
+
+Drop-in for scaled_vecchia.likelihood.vecchia_profile_loglik
+(reference likelihood.py:66-175) for fixed nu in {0.5, 1.5, 2.5}.  The
+structure (ordered inputs + neighbour groups) is uploaded once by
+`VecchiaStructure`; each evaluation is one pass of the fused HIP kernels.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["vecchia_profile_loglik", "VecchiaStructure"]
+
+_CHUNK_BUDGET = 4_000_000     # _covariance.py:255
+
+
+def _chunk_ids(groups, P):
+    """Jitter-chunk id of every block, in group order: _chunks(len(rows), K, P)
+    slicing of each group (_covariance.py:255-259, likelihood.py:111-114)."""
+    ids = []
+    base = 0
+    for rows, nb in groups:
+        K = nb.shape[1] + 1
+        step = max(1, _CHUNK_BUDGET // max(1, K * K * max(P, 1)))
+        local = np.arange(len(rows)) // step
+        ids.append(base + local)
+        base += int(local[-1]) + 1 if len(rows) else 0
+    return (np.concatenate(ids) if ids else np.zeros(0, np.int64)), base
+
+
+class VecchiaStructure:
+    """Device-resident Vecchia structure: ordered inputs, responses, design
+    and neighbour groups (the state gp.py:182-191 rebuilds)."""
+
+    def __init__(self, Xo, yo, Zo, groups, ctx=None):
+        self.ctx = ctx or _lib.context()
+        Xo = np.ascontiguousarray(Xo, dtype=np.float64)
+        yo = np.ascontiguousarray(yo, dtype=np.float64)
+        self.n, self.d = Xo.shape
+        q = 0 if (Zo is None or Zo.shape[1] == 0) else Zo.shape[1]
+        self.q = q
+        groups = [(np.asarray(r), np.asarray(nb)) for r, nb in groups]
+        mmax = max((nb.shape[1] for _, nb in groups), default=0)
+        nblk = sum(len(r) for r, _ in groups)
+        rows = np.empty(nblk, dtype=np.int64)
+        nbr = np.full((nblk, max(mmax, 1)), -1, dtype=np.int64)
+        kcnt = np.empty(nblk, dtype=np.int32)
+        o = 0
+        for r, nb in groups:
+            B, k = len(r), nb.shape[1]
+            rows[o:o + B] = r
+            nbr[o:o + B, :k] = nb
+            kcnt[o:o + B] = k
+            o += B
+        P = self.d + 2
+        cg, ncg = _chunk_ids(groups, P)
+        cv, ncv = _chunk_ids(groups, 1)
+        self.ctx.vecchia_setup(Xo, yo, None if q == 0 else Zo, rows, nbr, kcnt, max(mmax, 1),
+                               cg, cv, max(ncg, 1), max(ncv, 1))
+        self.nblk = nblk
+        self.mmax = mmax
+
+    def __call__(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
+                 precision="fp64"):
+        return self.ctx.vecchia_eval(theta, nu, need_grad=need_grad, var_penalty=var_penalty,
+                                     log_var_target=log_var_target, precision=precision)
+
+
+def _check_nu(nu):
+    if nu not in (0.5, 1.5, 2.5):
+        raise NotImplementedError(
+            "the GPU path implements the closed-form Matern nu in {0.5, 1.5, 2.5}")
+
+
+def vecchia_profile_loglik(theta, Xo, yo, Zo, groups, nu, estimate_nu=False,
+                           need_grad=True, var_penalty=0.0, log_var_target=0.0,
+                           precision="fp64"):
+    """Profile Vecchia loglikelihood, gradient and Fisher information
+    (likelihood.py:66-175).  Returns (ll, grad, fisher, beta), or
+    (ll, None, None, beta) when need_grad is False."""
+    if estimate_nu:
+        raise NotImplementedError("estimating nu (Bessel Matern) is not on the GPU path")
+    _check_nu(nu)
+    st = VecchiaStructure(Xo, yo, Zo, groups)
+    return st(theta, nu, need_grad=need_grad, var_penalty=var_penalty,
+              log_var_target=log_var_target, precision=precision)

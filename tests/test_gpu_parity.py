@@ -1,0 +1,268 @@
+"""GPU parity: libsvh.so (through the C ABI) against the NumPy oracle.
+
+Bars: ordering and neighbour sets bit-exact; float64 likelihood, gradient,
+Fisher and beta within 1e-10 relative (1e-9 for Fisher/gradient entries,
+relative to the largest entry); predictions within 1e-9 relative.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vecchia_np as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+# ------------------------------------------------------------------ (a)
+@pytest.mark.parametrize("n,d,seed", [(1, 3, 0), (2, 2, 1), (57, 3, 2), (300, 4, 3),
+                                      (2000, 8, 4), (5000, 3, 5)])
+def test_maximin_bit_exact(gpu, n, d, seed):
+    X = np.random.default_rng(seed).random((n, d)) / 0.3
+    first = O.maximin_first(X)
+    np.testing.assert_array_equal(gpu.maximin(X, first), O.maximin_order(X, first))
+
+
+def test_maximin_grid_ties_and_duplicates(gpu):
+    g = np.arange(9.0)
+    X = np.array([[a, b] for a in g for b in g])
+    X = np.vstack([X, X[:7]])                      # exact duplicates too
+    first = O.maximin_first(X)
+    np.testing.assert_array_equal(gpu.maximin(X, first), O.maximin_order(X, first))
+
+
+def test_maximin_large_prefix_and_property(gpu):
+    n, d = 50000, 8
+    X = np.random.default_rng(1002).random((n, d)) / (0.2 * math.sqrt(d))
+    first = O.maximin_first(X)
+    o = gpu.maximin(X, first)
+    assert np.array_equal(np.sort(o), np.arange(n))
+    ref = O.maximin_order(X, first, steps=1500)
+    np.testing.assert_array_equal(o[:1500], ref)
+    # selected distances to the chosen set never increase
+    nn = gpu.ordered_knn(X[o], 1)
+    r = np.sqrt(((X[o][1:] - X[o][nn[1:, 1]]) ** 2).sum(1))
+    assert np.all(np.diff(r) <= 1e-12)
+
+
+@pytest.mark.parametrize("n,d,m,start,seed", [(120, 3, 8, 0, 2), (2000, 4, 30, 0, 7),
+                                              (600, 8, 30, 400, 8), (60, 2, 6, 40, 4),
+                                              (300, 1, 5, 0, 9), (200, 5, 140, 0, 10)])
+def test_ordered_knn_bit_exact(gpu, n, d, m, start, seed):
+    X = np.random.default_rng(seed).random((n, d))
+    np.testing.assert_array_equal(gpu.ordered_knn(X, m, start), O.find_ordered_nn(X, m, start))
+
+
+def test_ordered_knn_ties(gpu):
+    g = np.arange(12.0)
+    X = np.array([[a, b] for a in g for b in g])
+    np.testing.assert_array_equal(gpu.ordered_knn(X, 10), O.find_ordered_nn(X, 10))
+
+
+@pytest.mark.parametrize("nref,nq,d,k", [(3000, 200, 4, 140), (500, 50, 10, 30), (141, 5, 3, 141)])
+def test_knn_query_bit_exact(gpu, nref, nq, d, k):
+    rng = np.random.default_rng(nref)
+    Xr, Xq = rng.random((nref, d)), rng.random((nq, d))
+    np.testing.assert_array_equal(gpu.knn_query(Xr, Xq, k), O.knn_query(Xr, Xq, k))
+
+
+# ------------------------------------------------------------------ (b)
+def _toy(n=60, d=3, seed=1):
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, d))
+    theta = np.array([math.log(1.7)] + [math.log(v) for v in (0.35, 0.9, 3.0, 0.5, 1.2)[:d]]
+                     + [math.log(0.05)])
+    y = rng.standard_normal(n)
+    return X, y, theta
+
+
+def _structure(X, theta, m):
+    d = X.shape[1]
+    Xs = X / np.exp(theta[1:1 + d])
+    order = O.maximin_order(Xs)
+    nn = O.find_ordered_nn(Xs[order], m)
+    return order, O.nn_groups(nn, m)
+
+
+@pytest.mark.parametrize("nu", [0.5, 1.5, 2.5])
+@pytest.mark.parametrize("trend", ["zero", "constant", "linear"])
+@pytest.mark.parametrize("m", [10, 59])
+def test_loglik_matches_oracle(gpu, nu, trend, m):
+    from pyscaledvecchia_amd.likelihood import vecchia_profile_loglik
+    X, y, theta = _toy()
+    order, groups = _structure(X, theta, m)
+    Z = O.design(X, trend)[order]
+    Zarg = Z if Z.shape[1] else None
+    args = (theta, X[order], y[order], Zarg, groups, nu)
+    kw = dict(var_penalty=1.0, log_var_target=0.1)
+    ll, g, F, b = vecchia_profile_loglik(*args, **kw)
+    ll0, g0, F0, b0 = O.vecchia_profile_loglik(*args, **kw)
+    assert abs(ll - ll0) <= 1e-10 * abs(ll0)
+    assert _rel(g, g0) < 1e-9
+    assert _rel(F, F0) < 1e-9
+    if b0.size:
+        assert _rel(b, b0) < 1e-9
+    ll1, g1, F1, b1 = vecchia_profile_loglik(*args, need_grad=False, **kw)
+    assert g1 is None and F1 is None
+    assert abs(ll1 - ll0) <= 1e-10 * abs(ll0)
+
+
+def test_loglik_exact_gp_at_full_conditioning(gpu):
+    from pyscaledvecchia_amd.likelihood import vecchia_profile_loglik
+    X, y, theta = _toy()
+    order, groups = _structure(X, theta, 59)
+    ll, _, F, _ = vecchia_profile_loglik(theta, X[order], y[order], None, groups, 2.5)
+    ll_e, F_e, _ = O.exact_loglik(X, y, theta, 2.5)
+    assert abs(ll - ll_e) < 1e-9 * abs(ll_e)
+    np.testing.assert_allclose(F, F_e, rtol=1e-7, atol=1e-9)
+
+
+def test_loglik_jitter_ladder_matches_oracle(gpu):
+    from pyscaledvecchia_amd.likelihood import vecchia_profile_loglik
+    rng = np.random.default_rng(21)
+    X = rng.random((300, 2))
+    X[150:160] = X[100:110]                      # duplicated inputs
+    y = rng.standard_normal(300)
+    theta = np.log([1.0, 0.3, 0.3, 1e-12])       # nugget at the clip floor
+    order, groups = _structure(X, theta, 12)
+    args = (theta, X[order], y[order], np.ones((300, 1)), groups, 2.5)
+    rep = {}
+    ll0, g0, F0, b0 = O.vecchia_profile_loglik(*args, report=rep)
+    assert any(p is not None for p in rep["jitter"])
+    ll, g, F, b = vecchia_profile_loglik(*args)
+    assert abs(ll - ll0) <= 1e-8 * abs(ll0)
+    assert _rel(g, g0) < 1e-6
+    assert _rel(F, F0) < 1e-6
+
+
+def test_loglik_c2_shape_against_oracle(gpu):
+    """Full C2 dimensions (d=8, linear trend, m=30) on a 6000-row slice."""
+    from pyscaledvecchia_amd import VecchiaStructure
+    rng = np.random.default_rng(1002)
+    n, d = 6000, 8
+    X = rng.random((n, d))
+    y = np.sin(2 * np.pi * X[:, 0] / 0.5) + 0.8 * np.sin(2 * np.pi * X[:, 1] / 0.7) \
+        + 0.01 * rng.standard_normal(n)
+    theta = np.concatenate([[0.0], np.full(d, math.log(0.2 * math.sqrt(d))), [math.log(0.1)]])
+    order, groups = _structure(X, theta, 30)
+    Z = O.design(X, "linear")[order]
+    st = VecchiaStructure(X[order], y[order], Z, groups)
+    ll, g, F, b = st(theta, 2.5, var_penalty=1.0, log_var_target=0.0)
+    ll0, g0, F0, b0 = O.vecchia_profile_loglik(theta, X[order], y[order], Z, groups, 2.5,
+                                               var_penalty=1.0, log_var_target=0.0)
+    assert abs(ll - ll0) <= 1e-10 * abs(ll0)
+    assert _rel(g, g0) < 1e-9 and _rel(F, F0) < 1e-9 and _rel(b, b0) < 1e-9
+
+
+# ------------------------------------------------------------------ (c)
+def test_predict_matches_oracle(gpu):
+    rng = np.random.default_rng(31)
+    n, d, ns, m = 2000, 4, 300, 140
+    X = rng.random((n, d))
+    y = np.sin(4 * X[:, 0]) + X[:, 1] * X[:, 2]
+    gpo = O.OracleGP(m_est=10, m_pred=m, n_est=n, max_iter=1, var_correction=False).fit(X, y)
+    resid = gpo.y_ - O.design(gpo.X_, gpo.trend) @ gpo.beta_
+    Xt = rng.random((ns, d))
+    mu0, va0 = gpo._predict_scaled(Xt, gpo.X_, resid, m)
+    mu, va, nb = gpu.predict(gpo.X_, resid, Xt, m, gpo.theta_, 2.5, return_nb=True)
+    np.testing.assert_array_equal(nb, O.knn_query(gpo.X_ / gpo.ranges_, Xt / gpo.ranges_, m))
+    assert _rel(mu, mu0) < 1e-9
+    assert _rel(va, va0) < 1e-8
+
+
+def test_block_columns_match_oracle(gpu):
+    rng = np.random.default_rng(32)
+    n, d, ns, m = 500, 3, 40, 30
+    X = rng.random((n, d))
+    y = np.cos(3 * X[:, 0]) + X[:, 2]
+    gpo = O.OracleGP(m_est=10, m_pred=m, n_est=n, max_iter=1, var_correction=False).fit(X, y)
+    Xs = rng.random((ns, d))
+    U_op, U_pp, perm, oo = gpo._joint_factor(Xs, m)
+    # same blocks through the GPU
+    Xall = np.vstack([gpo.X_, Xs])
+    Xsc = Xall / gpo.ranges_
+    order = np.concatenate([O.maximin_order(Xsc[:n]), O.maximin_order(Xsc[n:]) + n])
+    nn = O.find_ordered_nn(Xsc[order], m, start=n)
+    rows = np.arange(n, n + ns)
+    blk = np.concatenate([nn[rows, 1:m + 1], rows[:, None]], axis=1)
+    mask = np.ones((ns, m + 1))
+    mask[:, m] = 0.0
+    mask[:, :m] = (blk[:, :m] < n)
+    a = gpu.block_columns(Xall[order], blk, mask, gpo.theta_, 2.5)
+    U = np.zeros((n + ns, ns))
+    for j in range(ns):
+        U[blk[j], j] = a[j]
+    Uref = np.vstack([U_op.toarray(), U_pp.toarray()])
+    assert _rel(U, Uref) < 1e-9
+
+
+# ------------------------------------------------------------------ estimator
+def _sine(n, d=2, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, d))
+    return X, np.sin(3 * X[:, 0]) + 0.1 * X[:, 1]
+
+
+def test_fit_matches_oracle_fit(gpu):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    X, y = _sine(400, d=3, seed=5)
+    kw = dict(m_est=15, m_pred=30, n_est=300, nugget=1e-4, random_state=3)
+    g = ScaledVecchiaGP(**kw).fit(X, y)
+    o = O.OracleGP(**kw).fit(X, y)
+    assert abs(g.loglik_ - o.loglik_) <= 1e-8 * abs(o.loglik_)
+    np.testing.assert_allclose(g.theta_, o.theta_, rtol=1e-6, atol=1e-6)
+    assert g.b_ == pytest.approx(o.b_, rel=1e-6)
+    Xt = np.random.default_rng(6).random((50, 3))
+    mu, sd = g.predict(Xt, return_std=True)
+    mu0, sd0 = o.predict(Xt, return_std=True)
+    np.testing.assert_allclose(mu, mu0, rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(sd, sd0, rtol=1e-6)
+
+
+def test_reference_style_api(gpu):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    with pytest.raises(ValueError):
+        ScaledVecchiaGP().fit(np.random.rand(10, 2), np.random.rand(9))
+    with pytest.raises(RuntimeError):
+        ScaledVecchiaGP().predict(np.zeros((1, 2)))
+    with pytest.raises(ValueError):
+        ScaledVecchiaGP(trend="bogus").fit(np.random.rand(20, 2), np.random.rand(20))
+    X, y = _sine(300, seed=0)
+    gp = ScaledVecchiaGP(m_est=15, m_pred=30, nu=2.5, n_est=300).fit(X, y)
+    Xt, yt = _sine(200, seed=1)
+    mu, sd = gp.predict(Xt, return_std=True)
+    assert np.sqrt(np.mean((mu - yt) ** 2)) < 0.25 and np.all(sd >= 0)
+    mu2, var = gp.predict(Xt, return_var=True)
+    np.testing.assert_allclose(mu, mu2)
+    np.testing.assert_allclose(sd ** 2, var, rtol=1e-10)
+    np.testing.assert_allclose(gp.predict(Xt), mu)
+    assert "ScaledVecchiaGP" in gp.summary() and "loglik" in gp.summary()
+    g2 = ScaledVecchiaGP(m_est=10, m_pred=20, n_est=120, nugget=1e-8).fit(*_sine(120, seed=6))
+    assert g2.nugget_ == pytest.approx(1e-8, rel=1e-6)
+    assert np.all(g2.ranges_ > 0) and np.all(g2.relevance_ > 0) and g2.variance_ > 0
+
+
+@pytest.mark.parametrize("trend", ["zero", "constant", "linear"])
+def test_all_trends(gpu, trend):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    X, y = _sine(100, seed=13)
+    gp = ScaledVecchiaGP(m_est=8, m_pred=15, n_est=100, trend=trend).fit(X, y)
+    assert gp.predict(X[:5]).shape == (5,)
+
+
+def test_joint_prediction(gpu):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    X, y = _sine(200, seed=8)
+    gp = ScaledVecchiaGP(m_est=15, m_pred=30, n_est=200).fit(X, y)
+    Xt = np.random.default_rng(9).random((25, 2))
+    joint = gp.predict_joint(Xt, m=30, exact_var=True)
+    np.testing.assert_allclose(gp.predict(Xt), joint["mean"], atol=0.05)
+    assert np.all(joint["var"] >= 0)
+    s = gp.sample_joint(Xt[:12], n_sim=25, m=20)
+    assert s.shape == (25, 12) and np.all(np.isfinite(s))
