@@ -25,6 +25,8 @@
 #include "svh_internal.h"
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 const double kJitterPow10[10] = {0.0,  1e-12, 1e-11, 1e-10, 1e-09,
@@ -292,57 +294,419 @@ __global__ __launch_bounds__(kWave) void ll_pass1(LLArgs a) {
   for (int e = lane; e < NACC; e += kWave) a.partial[(int64_t)blockIdx.x * NACC + e] = acc[e];
 }
 
+// ---------------------------------------------------------------- pass 1, split rows
+// One conditioning block per wave for K <= 32, d <= 10, q <= 11.  Lane (r, h)
+// (r = lane & 31, h = lane >> 5) owns row r, columns 16h..16h+15 of the block
+// matrix in registers; rows K..31 are identity padding.
+//  * Pairs are split cyclically (row r takes (r, r+t mod 32), t = 0..16; half h
+//    takes t in [9h, 9h+9)), so each exp/sqrt runs once; Sigma goes to the lower
+//    triangle of the LDS scratch Mt, g(r) to the upper triangle.
+//  * Cholesky in column-scaled form: L'_ij = L_ij / L_jj (unit lower) and pivots
+//    p_j = L_jj^2; a step costs one reciprocal, no square root.  L' is stored
+//    column-major to LDS (Lc) as it is produced.
+//  * Right-hand sides live split by half: half 0 holds y and Z (slots 0..11),
+//    half 1 the derivative columns dS_p a (slot 0 sigma^2, 1..10 ranges,
+//    11 nugget).  The forward solve runs on the unit factor and scales rows by
+//    1/L_rr = rsqrt(p_r) at the end.
+constexpr int kSK = 32, kSH = 16, kSD = 10, kSQ = 11, kSC = 12;
+
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
+template <int NU>
+__device__ __forceinline__ void matern_t(double r, double& M, double& g) {
+  if constexpr (NU == 0) {
+    if (r > 0.0) {
+      double e = exp(-r);
+      M = e;
+      g = -e / r;
+    } else {
+      M = 1.0;
+      g = 0.0;
+    }
+  } else if constexpr (NU == 1) {
+    double c = 1.7320508075688772 * r;
+    double e = exp(-c);
+    M = (1.0 + c) * e;
+    g = -3.0 * e;
+  } else {
+    double c = 2.23606797749979 * r;
+    double e = exp(-c);
+    M = (1.0 + c + c * c * (1.0 / 3.0)) * e;
+    g = -(5.0 / 3.0) * (1.0 + c) * e;
+  }
+}
+
+// LDS doubles per wave: Cb 32 | Bc 24 | Misc 8 | Lc 32*32 | Xs 32*d | Mt 32*33
+__host__ __device__ inline int split_lds_doubles(int d) {
+  return 32 + 24 + 8 + 32 * 32 + 32 * d + 32 * 33;
+}
+
+template <int NU>
+__global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
+  const int d = a.d, q = a.q, P = a.P;
+  const bool grad = a.need_grad != 0;
+  double* Cb = lds;          // one column of the block matrix / the vector a
+  double* Bc = Cb + 32;      // one row of right-hand sides (24 slots)
+  double* Misc = Bc + 24;    // scalars of row K-1
+  double* Lc = Misc + 8;     // L' column-major: Lc[j*32 + i] = L'_ij (i > j)
+  double* Xs = Lc + 32 * 32; // 32 x d scaled coordinates
+  double* Mt = Xs + 32 * d;  // 32 x 33 scratch matrix
+  const int nF = grad ? P * (P + 1) / 2 : 0;
+  const int nT = grad ? P + P * q : 0;
+  const int nX = q * q + q;
+  const int NC = grad ? 2 + q + 2 * P + P * q : 2 + q;
+  const int dlo = h ? (d + 1) / 2 : 0, dhi = h ? d : (d + 1) / 2;   // this half's dims
+  double accF[2] = {0.0, 0.0};   // Fisher tasks lane, lane+64 (nF <= 78)
+  double accX[3] = {0.0, 0.0, 0.0};  // GLS tasks lane, +64, +128 (nX <= 132)
+
+  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x) {
+    const int K = a.kcnt[b] + 1;
+    const int ch = a.chunk[b];
+    const int lev = a.level[ch];
+    const bool act = r < K;
+    int pt = 0;
+    if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
+    __syncthreads();
+    for (int l = dlo; l < dhi; ++l)
+      Xs[r * d + l] = act ? a.X[(int64_t)pt * d + l] * a.inv_lam[l] : 0.0;
+    __syncthreads();
+    double xr[kSD];
+#pragma unroll
+    for (int l = 0; l < kSD; ++l) xr[l] = (l < d) ? Xs[r * d + l] : 0.0;
+    // ---- pairs: Sigma lower, g upper
+#pragma unroll 1
+    for (int t = 9 * h; t < 9 * h + 9 && t <= 16; ++t) {
+      const int c = (r + t) & 31;
+      double Mv = 0.0, gg = 0.0;
+      if (t == 0) {
+        Mv = act ? a.var * (1.0 + a.nug) : 1.0;
+      } else if (act && c < K) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int l = 0; l < kSD; ++l)
+          if (l < d) {
+            const double dd = xr[l] - Xs[c * d + l];
+            s2 = fma(dd, dd, s2);
+          }
+        matern_t<NU>(sqrt(s2), Mv, gg);
+        Mv *= a.var;
+      }
+      const int hi = r > c ? r : c, lo = r > c ? c : r;
+      Mt[hi * 33 + lo] = Mv;
+      if (t != 0) Mt[lo * 33 + hi] = gg;
+    }
+    __syncthreads();
+    double S[kSH];
+#pragma unroll
+    for (int cc = 0; cc < kSH; ++cc) {
+      const int c = kSH * h + cc;
+      S[cc] = (c <= r) ? Mt[r * 33 + c] : Mt[c * 33 + r];
+    }
+    // ---- chunk jitter (rare): Sigma += 10^p tr(Sigma)/K I on the real rows
+    double jit = 0.0;
+    if (lev > 0) {
+      __syncthreads();
+      if (h == 0) Cb[r] = act ? Mt[r * 33 + r] : 0.0;
+      __syncthreads();
+      double tr = 0.0;
+      for (int t = 0; t < K; ++t) tr += Cb[t];
+      jit = kJitDev[lev] * (tr / K);
+#pragma unroll
+      for (int cc = 0; cc < kSH; ++cc)
+        if (kSH * h + cc == r && act) S[cc] += jit;
+    }
+    // ---- Cholesky, column-scaled
+    bool fail = false;
+    double piv = 1.0;
+#pragma unroll
+    for (int k = 0; k < kSK; ++k) {
+      const int hk = k / kSH, kc = k % kSH;
+      __syncthreads();
+      if (h == hk) Cb[r] = S[kc];
+      __syncthreads();
+      const double pk = Cb[k];
+      fail |= (k < K) && !(pk > 0.0);
+      const double lp = Cb[r] * rcp_nr(pk);
+      if (r == k) piv = pk;
+      Lc[k * 32 + r] = lp;   // column k of L' (rows <= k hold junk, never read)
+      // trailing update of this lane's columns c > k (columns <= k hold junk)
+#pragma unroll
+      for (int cc = 0; cc < kSH; ++cc) {
+        if (k >= kSH && cc <= k - kSH) continue;   // no lane owns a live column
+        S[cc] = fma(-lp, Cb[kSH * h + cc], S[cc]);
+      }
+      // keep each step's update in place: without this the scheduler defers
+      // the FMAs across steps and holds every step's operands in registers
+#pragma unroll
+      for (int cc = 0; cc < kSH; ++cc) asm volatile("" : "+v"(S[cc]));
+    }
+    const double dinv = rsqrt_nr(piv);   // 1 / L_rr
+    if (r == K - 1 && h == 0) {
+      Misc[0] = dinv;
+      Misc[1] = piv;
+      Misc[2] = 0.5 * log(piv);
+    }
+    __syncthreads();
+    // ---- a = L^-T e_K:  L'^T at = e_K, a = at / L_KK  (both halves, same values)
+    double at = (r == K - 1) ? 1.0 : 0.0;
+#pragma unroll 4
+    for (int i = kSK - 1; i >= 0; --i) {
+      const double ai = __shfl(at, i, 32);
+      if (r < i) at = fma(-Lc[r * 32 + i], ai, at);
+    }
+    const double dK = Misc[0];
+    const double pK = Misc[1];
+    const double av = at * dK;
+    if (h == 0) Cb[r] = av;
+    __syncthreads();
+    // ---- right-hand sides
+    double R[kSC];
+#pragma unroll
+    for (int c = 0; c < kSC; ++c) R[c] = 0.0;
+    if (grad) {
+      // dS_{1+l} a for this half's dims, then half 1 gathers all of them
+      double accl[kSD];
+#pragma unroll
+      for (int l = 0; l < kSD; ++l) accl[l] = 0.0;
+#pragma unroll 2
+      for (int j = 0; j < kSK; ++j) {
+        const double gj = (j == r) ? 0.0 : ((j < r) ? Mt[j * 33 + r] : Mt[r * 33 + j]);
+        const double wj = gj * Cb[j];
+#pragma unroll
+        for (int l = 0; l < kSD; ++l)
+          if (l >= dlo && l < dhi) {
+            const double dd = xr[l] - Xs[j * d + l];
+            accl[l] = fma(wj * dd, dd, accl[l]);
+          }
+      }
+#pragma unroll
+      for (int l = 0; l < kSD; ++l) {
+        const double other = __shfl_xor(accl[l], 32, 64);
+        const double full = (l >= dlo && l < dhi) ? accl[l] : other;
+        if (l < d) R[1 + l] = h ? -a.var * full : R[1 + l];
+      }
+      if (h == 1) {
+        R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
+        R[kSC - 1] = a.var * a.nug * av;
+      }
+    }
+    if (h == 0 && act) {
+      R[0] = a.y[pt];
+#pragma unroll
+      for (int c = 0; c < kSQ; ++c)
+        if (c < q) R[1 + c] = a.Z[(int64_t)pt * q + c];
+    }
+    // ---- forward solve with the unit factor L', then scale by 1/L_rr
+    const int ncol = grad ? 2 : 1;   // halves that carry columns
+#pragma unroll 1
+    for (int j = 0; j < kSK - 1; ++j) {
+      __syncthreads();
+      if (r == j && h < ncol) {
+#pragma unroll
+        for (int c = 0; c < kSC; ++c) Bc[kSC * h + c] = R[c];
+      }
+      __syncthreads();
+      if (h < ncol) {
+        const double wj = (j < r) ? Lc[j * 32 + r] : 0.0;   // L'_rj
+#pragma unroll
+        for (int c = 0; c < kSC; ++c) R[c] = fma(-wj, Bc[kSC * h + c], R[c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kSC; ++c) R[c] *= dinv;
+    const bool ok = !fail;
+    if (fail && lane == 0) atomicOr(a.fail + ch, 1);
+    __syncthreads();
+    // ---- columns to LDS by parameter index: W_p (p < P), then yt, Zt_c
+    const int cyl = grad ? P : 0;
+    if (h == 1 && grad) {
+      Mt[0 * 33 + r] = R[0];
+#pragma unroll
+      for (int l = 0; l < kSD; ++l)
+        if (l < d) Mt[(1 + l) * 33 + r] = R[1 + l];
+      Mt[(d + 1) * 33 + r] = R[kSC - 1];
+    }
+    if (h == 0) {
+      Mt[cyl * 33 + r] = R[0];
+#pragma unroll
+      for (int c = 0; c < kSQ; ++c)
+        if (c < q) Mt[(cyl + 1 + c) * 33 + r] = R[1 + c];
+    }
+    __syncthreads();
+    const int km = K - 1;
+    double* rec = a.cache + b * NC;
+    for (int e = lane; e < 2 + q + (grad ? P : 0); e += kWave) {
+      double v;
+      if (e == 0)
+        v = Mt[cyl * 33 + km];
+      else if (e <= q)
+        v = Mt[(cyl + e) * 33 + km];
+      else if (e == q + 1)
+        v = Misc[2];
+      else
+        v = Mt[(e - q - 2) * 33 + km];
+      if (ok) rec[e] = v;
+    }
+    for (int e = lane; e < nF + nT; e += kWave) {
+      int c1, c2;
+      if (e < nF) {
+        pair_of(e, c1, c2);
+      } else if (e < nF + P) {
+        c1 = e - nF;
+        c2 = cyl;
+      } else {
+        const int rr = e - nF - P;
+        c1 = rr / q;
+        c2 = cyl + 1 + (rr - c1 * q);
+      }
+      double sdot = 0.0;
+      for (int i = 0; i < K; ++i) sdot = fma(Mt[c1 * 33 + i], Mt[c2 * 33 + i], sdot);
+      if (e < nF) {
+        const double f = sdot - 0.5 * Mt[c1 * 33 + km] * Mt[c2 * 33 + km];
+        if (ok) {
+          if (e < kWave)
+            accF[0] += f;
+          else
+            accF[1] += f;
+        }
+      } else if (e < nF + P) {
+        if (ok) rec[2 + q + P + (e - nF)] = sdot;
+      } else {
+        if (ok) rec[2 + q + 2 * P + (e - nF - P)] = sdot;
+      }
+    }
+    for (int e = lane; e < nX; e += kWave) {
+      double v;
+      if (e < q * q) {
+        const int c1 = e / q, c2 = e - c1 * q;
+        v = Mt[(cyl + 1 + c1) * 33 + km] * Mt[(cyl + 1 + c2) * 33 + km];
+      } else {
+        const int c = e - q * q;
+        v = Mt[(cyl + 1 + c) * 33 + km] * Mt[cyl * 33 + km];
+      }
+      if (ok) {
+        if (e < kWave)
+          accX[0] += v;
+        else if (e < 2 * kWave)
+          accX[1] += v;
+        else
+          accX[2] += v;
+      }
+    }
+  }
+  // this workgroup's partials (layout of ll_pass1: F (P*P), XtX (q*q), Zty (q))
+  const int NACC = grad ? P * P + q * q + q : q * q + q;
+  const int X0 = grad ? P * P : 0;
+  double* part = a.partial + (int64_t)blockIdx.x * NACC;
+  for (int e = lane; e < nF; e += kWave) {
+    int c1, c2;
+    pair_of(e, c1, c2);
+    const double f = (e < kWave) ? accF[0] : accF[1];
+    part[c1 * P + c2] = f;
+    part[c2 * P + c1] = f;
+  }
+  for (int e = lane; e < nX; e += kWave)
+    part[X0 + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
+}
+
+template <int NU>
+void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a) {
+  hipLaunchKernelGGL((ll_pass1_split<NU>), dim3(G), dim3(kWave), lds, st, a);
+}
+
 // ---------------------------------------------------------------- reduce 1
-// Sum the pass-1 partials in fixed order; beta = solve(XtX + 1e-12 I, Zty).
-// out layout: [0] ll, [1..P] grad, [1+P .. 1+P+P*P) Fisher, then beta[q],
-// then nfail.
-__global__ __launch_bounds__(256) void ll_reduce1(const double* __restrict__ partial, int G,
-                                                  int NACC, int P, int q, int need_grad,
-                                                  double* __restrict__ tot,
-                                                  double* __restrict__ beta,
-                                                  double* __restrict__ out) {
-  for (int e = threadIdx.x; e < NACC; e += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < G; ++g) s += partial[(int64_t)g * NACC + e];
-    tot[e] = s;
+// (a) one workgroup per accumulator entry: fixed-order strided sums of the
+//     pass-1 partials, then an LDS tree;
+// (b) one workgroup: Fisher to the output, beta = solve(XtX + 1e-12 I, Zty) by
+//     Gaussian elimination with partial pivoting, parallel over the trailing
+//     submatrix in LDS.
+// out layout: [0] ll, [1..P] grad, [1+P .. 1+P+P*P) Fisher, then beta[q], nfail.
+__global__ __launch_bounds__(256) void ll_sum_partials(const double* __restrict__ partial, int G,
+                                                       int NACC, double* __restrict__ tot) {
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  double s = 0.0;
+  for (int g = threadIdx.x; g < G; g += 256) s += partial[(int64_t)g * NACC + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tot[e] = red[0];
+}
+
+__global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ tot, int P, int q,
+                                                     int need_grad, double* __restrict__ beta,
+                                                     double* __restrict__ out) {
+  __shared__ double A[32][34];
+  __shared__ int piv_s;
+  const int X0 = need_grad ? P * P : 0, Y0 = X0 + q * q;
+  const int t = threadIdx.x;
+  if (need_grad)
+    for (int e = t; e < P * P; e += 256) out[1 + P + e] = tot[e];
+  if (q == 0) return;
+  for (int e = t; e < q * (q + 1); e += 256) {
+    int i = e / (q + 1), j = e - i * (q + 1);
+    A[i][j] = (j < q) ? tot[X0 + i * q + j] + (i == j ? 1e-12 : 0.0) : tot[Y0 + i];
   }
   __syncthreads();
-  const int X0 = need_grad ? P * P : 0, Y0 = X0 + q * q;
-  if (need_grad)
-    for (int e = threadIdx.x; e < P * P; e += blockDim.x) out[1 + P + e] = tot[e];
-  if (threadIdx.x == 0 && q > 0) {
-    // Gaussian elimination with partial pivoting on [XtX + 1e-12 I | Zty]
-    double A[32][33];
-    for (int i = 0; i < q; ++i) {
-      for (int j = 0; j < q; ++j) A[i][j] = tot[X0 + i * q + j] + (i == j ? 1e-12 : 0.0);
-      A[i][q] = tot[Y0 + i];
-    }
-    for (int k = 0; k < q; ++k) {
-      int piv = k;
+  for (int k = 0; k < q; ++k) {
+    if (t == 0) {
+      int p = k;
       double best = fabs(A[k][k]);
       for (int i = k + 1; i < q; ++i)
         if (fabs(A[i][k]) > best) {
           best = fabs(A[i][k]);
-          piv = i;
+          p = i;
         }
-      if (piv != k)
-        for (int j = 0; j <= q; ++j) {
-          double t = A[k][j];
-          A[k][j] = A[piv][j];
-          A[piv][j] = t;
-        }
-      double inv = 1.0 / A[k][k];
-      for (int i = k + 1; i < q; ++i) {
-        double l = A[i][k] * inv;
-        for (int j = k; j <= q; ++j) A[i][j] -= l * A[k][j];
-      }
+      piv_s = p;
     }
+    __syncthreads();
+    const int p = piv_s;
+    if (p != k)
+      for (int j = t; j <= q; j += 256) {
+        double tmp = A[k][j];
+        A[k][j] = A[p][j];
+        A[p][j] = tmp;
+      }
+    __syncthreads();
+    const int rem = q - 1 - k, w = q + 1 - k;
+    for (int e = t; e < rem * w; e += 256) {
+      int i = k + 1 + e / w, j = k + (e % w);
+      if (j == k) continue;
+      A[i][j] -= (A[i][k] / A[k][k]) * A[k][j];
+    }
+    __syncthreads();
+    for (int i = k + 1 + t; i < q; i += 256) A[i][k] = 0.0;
+    __syncthreads();
+  }
+  if (t == 0) {
     for (int i = q - 1; i >= 0; --i) {
       double s = A[i][q];
-      for (int j = i + 1; j < q; ++j) s -= A[i][j] * beta[j];
-      beta[i] = s / A[i][i];
+      for (int j = i + 1; j < q; ++j) s -= A[i][j] * A[j][33];
+      A[i][33] = s / A[i][i];
     }
-    for (int i = 0; i < q; ++i) out[1 + P + P * P + i] = beta[i];
+    for (int i = 0; i < q; ++i) {
+      beta[i] = A[i][33];
+      out[1 + P + P * P + i] = A[i][33];
+    }
   }
 }
 
@@ -391,19 +755,22 @@ __global__ __launch_bounds__(256) void ll_reduce2(const double* __restrict__ par
                                                   int NV, const int* __restrict__ fail,
                                                   int nchunk, int P, int q,
                                                   double* __restrict__ out) {
-  __shared__ int nf;
-  if (threadIdx.x == 0) nf = 0;
-  __syncthreads();
-  for (int e = threadIdx.x; e < NV; e += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < G2; ++g) s += partial2[(int64_t)g * NV + e];
-    out[e] = s;
+  // one workgroup per output entry (blockIdx.x < NV) plus one for the failed-chunk count
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  double s = 0.0;
+  if (e < NV) {
+    for (int g = threadIdx.x; g < G2; g += 256) s += partial2[(int64_t)g * NV + e];
+  } else {
+    for (int c = threadIdx.x; c < nchunk; c += 256) s += (fail[c] != 0) ? 1.0 : 0.0;
   }
-  int loc = 0;
-  for (int c = threadIdx.x; c < nchunk; c += blockDim.x) loc += fail[c] != 0;
-  atomicAdd(&nf, loc);
+  red[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[1 + P + P * P + q] = (double)nf;
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[e < NV ? e : 1 + P + P * P + q] = red[0];
 }
 
 __global__ void ll_escalate(int* __restrict__ level, int* __restrict__ fail, int nchunk) {
@@ -521,7 +888,13 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   const int NACC = need_grad ? P * P + q * q + q : q * q + q;
   const int NC = need_grad ? 2 + q + 2 * P + P * q : 2 + q;
   const int NV = need_grad ? 1 + P : 1;
-  const int G = (int)std::min<int64_t>(vs.nblk, 2048);
+  // split-row kernel (one block per wave, rows in registers) when the block fits
+  const char* kenv = std::getenv("SVH_LL_KERNEL");
+  const bool use_split = !(kenv && std::string(kenv) == "lds") && vs.mmax + 1 <= kSK &&
+                         d <= kSD && q <= kSQ;
+  const size_t lds_split = (size_t)split_lds_doubles(d) * sizeof(double);
+  const int G = use_split ? (int)std::min<int64_t>(vs.nblk, 8192)
+                          : (int)std::min<int64_t>(vs.nblk, 2048);
   const int G2 = (int)std::min<int64_t>((vs.nblk + 255) / 256, 512);
   const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC) * sizeof(double) +
                      (size_t)KP * sizeof(int);
@@ -559,19 +932,28 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   int rounds = 0;
   for (;;) {
     ctx->timers.begin(s, "vecchia_pass1");
-    switch (KP) {
-      case 32: launch_pass1<32>(s, G, lds, a); break;
-      case 64: launch_pass1<64>(s, G, lds, a); break;
-      default: launch_pass1<128>(s, G, lds, a); break;
+    if (use_split) {
+      switch (nu_code) {
+        case 0: launch_split_nu<0>(s, G, lds_split, a); break;
+        case 1: launch_split_nu<1>(s, G, lds_split, a); break;
+        default: launch_split_nu<2>(s, G, lds_split, a); break;
+      }
+    } else {
+      switch (KP) {
+        case 32: launch_pass1<32>(s, G, lds, a); break;
+        case 64: launch_pass1<64>(s, G, lds, a); break;
+        default: launch_pass1<128>(s, G, lds, a); break;
+      }
     }
     SVH_HIP(hipGetLastError());
     ctx->timers.end(s);
-    hipLaunchKernelGGL(ll_reduce1, dim3(1), dim3(256), 0, s, a.partial, G, NACC, P, q, need_grad,
-                       tot, dbeta, dout);
+    if (NACC > 0)
+      hipLaunchKernelGGL(ll_sum_partials, dim3(NACC), dim3(256), 0, s, a.partial, G, NACC, tot);
+    hipLaunchKernelGGL(ll_solve_beta, dim3(1), dim3(256), 0, s, tot, P, q, need_grad, dbeta, dout);
     hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), 0, s, a.cache, vs.nblk, P, q, need_grad,
                        dbeta, partial2);
-    hipLaunchKernelGGL(ll_reduce2, dim3(1), dim3(256), 0, s, partial2, G2, NV, fail, nchunk, P, q,
-                       dout);
+    hipLaunchKernelGGL(ll_reduce2, dim3(NV + 1), dim3(256), 0, s, partial2, G2, NV, fail, nchunk,
+                       P, q, dout);
     SVH_HIP(hipGetLastError());
     SVH_HIP(hipMemcpyAsync(hout, dout, NOUT * sizeof(double), hipMemcpyDeviceToHost, s));
     SVH_HIP(hipStreamSynchronize(s));

@@ -44,6 +44,59 @@ __device__ __forceinline__ void wave_argmax(double& k, int& i) {
   }
 }
 
+__device__ __forceinline__ void wave_argmax3(double& k, int& i, int& e) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    double k2 = __shfl_xor(k, off, kWave);
+    int i2 = __shfl_xor(i, off, kWave);
+    int e2 = __shfl_xor(e, off, kWave);
+    if (mm_greater(k2, i2, k, i)) {
+      k = k2;
+      i = i2;
+      e = e2;
+    }
+  }
+}
+
+// Full-wave maximum of an unsigned value with DPP moves (result in every lane):
+// quad swaps, row shifts with bank masks, row broadcasts; lane 63 ends with
+// the maximum, which readlane broadcasts.  Every lane must be active.
+__device__ __forceinline__ unsigned dpp_wave_max(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xE, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xC, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Order-preserving map of a double to an unsigned 64-bit key.
+__device__ __forceinline__ unsigned long long ord_key(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Exact wave arg-max of (key desc, idx asc) by three DPP reductions: high
+// word of the key, low word among the tied, then the smallest index among
+// exact ties.  Returns the winning idx and the winner lane's `e`.
+__device__ __forceinline__ void wave_argmax_dpp(double k, int i, int e, double& bk, int& bi,
+                                                int& be) {
+  const unsigned long long u = ord_key(k);
+  const unsigned hi = (unsigned)(u >> 32), lo = (unsigned)u;
+  const unsigned m1 = dpp_wave_max(hi);
+  const unsigned m2 = dpp_wave_max(hi == m1 ? lo : 0u);
+  const bool tie = (hi == m1) && (lo == m2);
+  const unsigned mi = dpp_wave_max(tie ? ~(unsigned)i : 0u);
+  bi = (int)~mi;
+  const unsigned long long win = __ballot(tie && i == bi);
+  const int wl = (int)__ffsll((long long)win) - 1;
+  be = __builtin_amdgcn_readlane(e, wl);
+  const unsigned long long ku = ((unsigned long long)m1 << 32) | m2;
+  const unsigned long long kb = (ku >> 63) ? (ku & 0x7fffffffffffffffull) : ~ku;
+  bk = __longlong_as_double((long long)kb);
+}
+
 // ------------------------------------------------------------------ transpose
 // XT[l*n + j] = X[j*d + l] * (scale ? 1 : 1) / lam[l] (lam may be NULL).
 __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
@@ -80,6 +133,7 @@ constexpr int kSliceTop = 4;          // published candidates per slice
 constexpr int kPool = 256;            // pool entries (4 per lane)
 constexpr int kAccCap = 256;          // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
+constexpr int kMMD = 12;              // max d of the maximin kernels
 
 struct MMState {
   int count;  // points placed in `order`
@@ -171,11 +225,10 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
     const double* __restrict__ XT, int n, int d, const double* __restrict__ s2g,
     int nslice, const double* __restrict__ cand_key, const int* __restrict__ cand_idx,
     int* __restrict__ acc, MMState* __restrict__ st, int64_t* __restrict__ order, int sortcap) {
-  // LDS: survivors (sortcap keys + sortcap indices), then pool coordinates
+  // LDS: survivors (sortcap keys + sortcap indices)
   extern __shared__ double smem[];
   double* sk = smem;
   int* si = reinterpret_cast<int*>(smem + sortcap);
-  double* poolX = smem + sortcap + (sortcap + 1) / 2;
   const int lane = threadIdx.x;
   const int count0 = st->count;
   if (count0 >= n) {
@@ -270,36 +323,45 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
     if (lane == 0) st->overflow_sorts += 1;
     __syncthreads();
   }
-  // load pool: kPool / 64 slots per lane
-  double pv[kPool / kWave], ps[kPool / kWave];
-  int pi[kPool / kWave];
+  // load pool: kPool / 64 slots per lane, coordinates in registers
+  constexpr int kNT = kPool / kWave;
+  double pv[kNT], ps[kNT], px[kNT][kMMD];
+  int pi[kNT];
 #pragma unroll
-  for (int t = 0; t < kPool / kWave; ++t) {
+  for (int t = 0; t < kNT; ++t) {
     int e = lane + t * kWave;
     if (e < nsurv) {
       pv[t] = sk[e];
       pi[t] = si[e];
       ps[t] = s2g[pi[t]];
-      for (int l = 0; l < d; ++l) poolX[e * d + l] = XT[(int64_t)l * n + pi[t]];
     } else {
       pv[t] = -3.0;
       pi[t] = INT_MAX;
       ps[t] = 0.0;
     }
+#pragma unroll
+    for (int l = 0; l < kMMD; ++l)
+      px[t][l] = (l < d && e < nsurv) ? XT[(int64_t)l * n + pi[t]] : 0.0;
   }
-  __syncthreads();
   int nacc = 0;
   const int room = n - count0;
   while (nacc < kAccCap && nacc < room) {
-    double bk = -3.0;
-    int bi = INT_MAX;
+    // this lane's best slot, with its coordinates
+    double lk = -3.0, lx[kMMD];
+    int li = INT_MAX;
 #pragma unroll
-    for (int t = 0; t < kPool / kWave; ++t)
-      if (mm_greater(pv[t], pi[t], bk, bi)) {
-        bk = pv[t];
-        bi = pi[t];
-      }
-    wave_argmax(bk, bi);
+    for (int l = 0; l < kMMD; ++l) lx[l] = 0.0;
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      const bool b = mm_greater(pv[t], pi[t], lk, li);
+      lk = b ? pv[t] : lk;
+      li = b ? pi[t] : li;
+#pragma unroll
+      for (int l = 0; l < kMMD; ++l) lx[l] = b ? px[t][l] : lx[l];
+    }
+    double bk;
+    int bi, wl;
+    wave_argmax_dpp(lk, li, lane, bk, bi, wl);
     if (bk < 0.0) break;
     if (nacc > 0 && !mm_greater(bk, bi, Vk, Vi)) break;
     if (lane == 0) {
@@ -307,8 +369,17 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
       order[count0 + nacc] = bi;
     }
     ++nacc;
+    // coordinates of the new point from the winning lane (scalar registers)
+    double sx[kMMD];
 #pragma unroll
-    for (int t = 0; t < kPool / kWave; ++t) {
+    for (int l = 0; l < kMMD; ++l) {
+      const long long v = __double_as_longlong(lx[l]);
+      const int vlo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), wl);
+      const int vhi = __builtin_amdgcn_readlane((int)(v >> 32), wl);
+      sx[l] = __longlong_as_double(((long long)vhi << 32) | (unsigned)vlo);
+    }
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
       if (pi[t] == bi) {
         pv[t] = -3.0;
         continue;
@@ -318,9 +389,10 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
         pv[t] = -3.0;
         continue;
       }
-      int e = lane + t * kWave;
       double s = 0.0;
-      for (int l = 0; l < d; ++l) s = sq_step(s, poolX[e * d + l], XT[(int64_t)l * n + bi]);
+#pragma unroll
+      for (int l = 0; l < kMMD; ++l)
+        if (l < d) s = sq_step(s, px[t][l], sx[l]);
       if (s < ps[t]) {
         ps[t] = s;
         pv[t] = sqrt(s);
@@ -514,20 +586,28 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
   while (sortcap < nslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
   const size_t select_lds =
-      (size_t)sortcap * sizeof(double) + (size_t)((sortcap + 1) / 2) * sizeof(double) +
-      (size_t)kPool * d * sizeof(double);
+      (size_t)sortcap * sizeof(double) + (size_t)((sortcap + 1) / 2) * sizeof(double);
   SVH_REQUIRE(select_lds <= 150 * 1024, "maximin: d too large for the selection pool");
-  SVH_REQUIRE(apply_lds <= 64 * 1024, "maximin: d too large");
+  SVH_REQUIRE(apply_lds <= 64 * 1024 && d <= kMMD, "maximin: d above 12 not supported");
   MMState* hst = static_cast<MMState*>(ctx->stage.get(sizeof(MMState)));
   // enqueue rounds in batches; each round accepts at least one point
   int batch = 8;
-  ctx->timers.begin(ctx->stream, "maximin");
+  hipEvent_t mm_t0 = nullptr, mm_t1 = nullptr;
+  if (ctx->timers.enabled) {
+    SVH_HIP(hipEventCreate(&mm_t0));
+    SVH_HIP(hipEventCreate(&mm_t1));
+    SVH_HIP(hipEventRecord(mm_t0, ctx->stream));
+  }
   for (;;) {
     for (int r = 0; r < batch; ++r) {
+      ctx->timers.begin(ctx->stream, "maximin_apply");
       hipLaunchKernelGGL(mm_apply_kernel, dim3(nslice), dim3(kSlice), apply_lds, ctx->stream,
                          dXT, n, d, s2, dist, acc, st, ck, ci);
+      ctx->timers.end(ctx->stream);
+      ctx->timers.begin(ctx->stream, "maximin_select");
       hipLaunchKernelGGL(mm_select_kernel, dim3(1), dim3(kWave), select_lds, ctx->stream, dXT,
                          n, d, s2, nslice, ck, ci, acc, st, dorder, sortcap);
+      ctx->timers.end(ctx->stream);
     }
     SVH_HIP(hipGetLastError());
     SVH_HIP(hipMemcpyAsync(hst, st, sizeof(MMState), hipMemcpyDeviceToHost, ctx->stream));
@@ -535,7 +615,17 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
     if (hst->count >= n) break;
     if (batch < 64) batch *= 2;
   }
-  ctx->timers.end(ctx->stream);
+  if (mm_t0) {
+    SVH_HIP(hipEventRecord(mm_t1, ctx->stream));
+    SVH_HIP(hipEventSynchronize(mm_t1));
+    float ms = 0.f;
+    SVH_HIP(hipEventElapsedTime(&ms, mm_t0, mm_t1));
+    auto& acc_t = ctx->timers.acc["maximin"];
+    acc_t.ms += ms;
+    acc_t.count += 1;
+    (void)hipEventDestroy(mm_t0);
+    (void)hipEventDestroy(mm_t1);
+  }
   SVH_HIP(hipStreamSynchronize(ctx->stream));
   ctx->timers.collect();
   ctx->mm_rounds = hst->rounds;
