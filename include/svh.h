@@ -66,6 +66,15 @@ int svh_ordered_knn(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t
 int svh_knn_query(svh_ctx* ctx, const double* Xref, int64_t nref, const double* Xq,
                   int64_t nq, int32_t d, int32_t k, int64_t* nb);
 
+/* Several datasets at once: for dataset i (rows n[i] x d[i] of X, stored back
+ * to back), the maximin order from first[i] (svh_maximin) and the ordered
+ * neighbours of the reordered rows (svh_ordered_knn with start 0).  Orders are
+ * dataset-local, written back to back to order; nn rows (m+1 wide) likewise.
+ * Replaces the rebuild of gp.py:182-191 for many fits at once (d <= 12). */
+int svh_ordering_batch(svh_ctx* ctx, int32_t ndata, const int64_t* n, const int32_t* d,
+                       const int64_t* first, const double* X, int32_t m, int64_t* order,
+                       int64_t* nn);
+
 /* ---------------------------------------------------------------- (b) --- */
 
 /* Upload one Vecchia structure (inputs already in Vecchia order).
@@ -115,6 +124,44 @@ int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t nt
 int svh_block_columns(svh_ctx* ctx, const double* Xall, int64_t nall, int32_t d,
                       const int64_t* blocks, const double* mask, int64_t nblk, int32_t K,
                       const double* theta, double nu, int32_t precision, double* a);
+
+/* ------------------------------------------------------------ batched --- */
+/* Many independent fits at once (no counterpart in the reference, which fits
+ * one dataset per ScaledVecchiaGP.fit, gp.py:138-239).  All blocks of all
+ * evaluating datasets go through one launch per kernel; a per-dataset
+ * Fisher-scoring state machine on the device restates optimize.py:11-73
+ * (ridge-until-ascent step, step cap 3, <= 20 strict-increase halvings, stop
+ * on |alpha step . g| < tol, reorder pause at k = 2, 4, 8, ...).
+ * Limits: d <= 10, q <= 11, m <= 31 (K <= 32).  Parameter arrays use a stride
+ * of 12 per dataset, beta a stride of 11, Fisher 144.
+ *   setup:     concatenated inputs of ndata datasets already in Vecchia order;
+ *              n[i] rows each; nbr/kcnt/chunk are per row (dataset-local
+ *              neighbour indices, stride mmax; chunk ids local to the dataset
+ *              as _covariance.py:255-259 slices them).  lvt = log variance
+ *              target of the sigma^2 penalty; fixed_nug/log_nug hold a fixed
+ *              nugget (gp.py:196-208).
+ *   update:    replace dataset i's ordered inputs/neighbours after a rebuild.
+ *   fit_start: initial log-parameters, state machines reset; use_graph != 0
+ *              captures one step (evaluation + update) in a hipGraph.
+ *   fit_step:  replays nrep steps; status[i] = 0 evaluating, 1 searching,
+ *              2 reorder due (host decides with svh_batch_set_reorder),
+ *              3 done, 4 failed (Cholesky at the start / after a rebuild).
+ *   eval:      one objective evaluation of every dataset at theta. */
+int svh_batch_setup(svh_ctx* ctx, int32_t ndata, const int32_t* d, const int32_t* q,
+                    const int32_t* nu_code, const int64_t* n, const double* X, const double* y,
+                    const double* Z, const int64_t* nbr, const int32_t* kcnt,
+                    const int32_t* chunk, const int32_t* nchunk, int32_t mmax, const double* lvt,
+                    const int32_t* fixed_nug, const double* log_nug, double lambda_max);
+int svh_batch_update(svh_ctx* ctx, int32_t i, const double* X, const double* y, const double* Z,
+                     const int64_t* nbr, const int32_t* kcnt);
+int svh_batch_fit_start(svh_ctx* ctx, const double* theta0, int32_t max_iter, double tol,
+                        double var_penalty, int32_t use_graph);
+int svh_batch_fit_step(svh_ctx* ctx, int32_t nrep, int32_t* status);
+int svh_batch_get(svh_ctx* ctx, double* theta, double* ll, double* beta, int32_t* status,
+                  int32_t* iters, int32_t* nevals);
+int svh_batch_set_reorder(svh_ctx* ctx, int32_t i, int32_t rebuilt);
+int svh_batch_eval(svh_ctx* ctx, const double* theta, double* ll, double* grad, double* fisher,
+                   double* beta, int32_t* jitter_rounds);
 
 /* ------------------------------------------------------- instrumentation --- */
 /* No counterpart in the reference (which has no timers).  When enabled, the

@@ -64,7 +64,21 @@ def _load():
     L.svh_timing_enable.argtypes = [_vp, ctypes.c_int32]
     L.svh_timing_read.argtypes = [_vp, ctypes.c_char_p, _dp, _i64p]
     L.svh_timing_reset.argtypes = [_vp]
-    for name in ("svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
+    L.svh_ordering_batch.argtypes = [_vp, ctypes.c_int32, _i64p, _i32p, _i64p, _dp,
+                                     ctypes.c_int32, _i64p, _i64p]
+    L.svh_batch_setup.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _i64p, _dp, _dp, _dp,
+                                  _i64p, _i32p, _i32p, _i32p, ctypes.c_int32, _dp, _i32p, _dp,
+                                  ctypes.c_double]
+    L.svh_batch_update.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _i64p, _i32p]
+    L.svh_batch_fit_start.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int32]
+    L.svh_batch_fit_step.argtypes = [_vp, ctypes.c_int32, _i32p]
+    L.svh_batch_get.argtypes = [_vp, _dp, _dp, _dp, _i32p, _i32p, _i32p]
+    L.svh_batch_set_reorder.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
+    L.svh_batch_eval.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _i32p]
+    for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_fit_start", "svh_batch_fit_step",
+                 "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
+                 "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
                  "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -74,7 +88,9 @@ def _load():
 EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh_last_error",
             "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
             "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_timing_enable",
-            "svh_timing_read", "svh_timing_reset")
+            "svh_timing_read", "svh_timing_reset", "svh_batch_setup", "svh_batch_update",
+            "svh_batch_fit_start", "svh_batch_fit_step", "svh_batch_get", "svh_batch_set_reorder",
+            "svh_batch_eval", "svh_ordering_batch")
 
 
 def device_count() -> int:
@@ -168,6 +184,24 @@ class Context:
                                           Xq.shape[0], Xref.shape[1], int(k), _ptr(nb, _i64p)))
         return nb
 
+    def ordering_batch(self, Xs, m):
+        """Maximin orders and ordered m-NN of several datasets in one pass."""
+        Xs = [_finite(X, "X") for X in Xs]
+        n = np.array([X.shape[0] for X in Xs], np.int64)
+        d = np.array([X.shape[1] for X in Xs], np.int32)
+        first = np.array([int(np.argmin(((X - X.mean(0)) ** 2).sum(1))) for X in Xs], np.int64)
+        Xc = np.concatenate([X.ravel() for X in Xs])
+        order = np.empty(int(n.sum()), np.int64)
+        nn = np.empty((int(n.sum()), m + 1), np.int64)
+        self._check(self._L.svh_ordering_batch(self._h, len(Xs), _ptr(n, _i64p), _ptr(d, _i32p),
+                                               _ptr(first, _i64p), _ptr(Xc), int(m),
+                                               _ptr(order, _i64p), _ptr(nn, _i64p)))
+        out, o = [], 0
+        for k in n:
+            out.append((order[o:o + k], nn[o:o + k]))
+            o += k
+        return out
+
     # ---------------------------------------------------------------- (b)
     def vecchia_setup(self, Xo, yo, Zo, rows, nbr, kcnt, mmax, chunk_grad, chunk_val,
                       nchunk_grad, nchunk_val):
@@ -228,6 +262,91 @@ class Context:
             self._h, _ptr(Xall), Xall.shape[0], Xall.shape[1], _ptr(blocks, _i64p), _ptr(mask),
             nblk, K, _ptr(theta), float(nu), PRECISION[precision], _ptr(a)))
         return a
+
+
+BP, BQ = 12, 11   # parameter / trend strides of the batched entry points
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class BatchHandle:
+    """Thin wrapper of the svh_batch_* entry points on one context."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        self._L = ctx._L
+
+    def setup(self, d, q, nu_code, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt, fixed_nug,
+              log_nug, lambda_max):
+        self.n = _i64(n)
+        self.ndata = len(self.n)
+        X, y = _finite(X, "X"), _finite(y, "y")
+        Z = _f64(Z) if Z is not None and Z.size else np.zeros(1)
+        args = [_i32(d), _i32(q), _i32(nu_code), self.n]
+        self._keep = args
+        c = self.ctx
+        c._check(self._L.svh_batch_setup(
+            c._h, self.ndata, _ptr(args[0], _i32p), _ptr(args[1], _i32p), _ptr(args[2], _i32p),
+            _ptr(args[3], _i64p), _ptr(X), _ptr(y), _ptr(Z), _ptr(_i64(nbr), _i64p),
+            _ptr(_i32(kcnt), _i32p), _ptr(_i32(chunk), _i32p), _ptr(_i32(nchunk), _i32p),
+            int(mmax), _ptr(_f64(lvt)), _ptr(_i32(fixed_nug), _i32p), _ptr(_f64(log_nug)),
+            float(lambda_max)))
+
+    def update(self, i, X, y, Z, nbr, kcnt):
+        Z = _f64(Z) if Z is not None and Z.size else np.zeros(1)
+        nbr, kcnt = _i64(nbr), _i32(kcnt)
+        c = self.ctx
+        c._check(self._L.svh_batch_update(c._h, int(i), _ptr(_finite(X, "X")),
+                                          _ptr(_finite(y, "y")), _ptr(Z), _ptr(nbr, _i64p),
+                                          _ptr(kcnt, _i32p)))
+
+    def fit_start(self, theta0, max_iter, tol, var_penalty, use_graph=True):
+        th = _f64(theta0)
+        c = self.ctx
+        c._check(self._L.svh_batch_fit_start(c._h, _ptr(th), int(max_iter), float(tol),
+                                             float(var_penalty), int(bool(use_graph))))
+
+    def fit_step(self, nrep=1):
+        st = np.zeros(self.ndata, dtype=np.int32)
+        c = self.ctx
+        c._check(self._L.svh_batch_fit_step(c._h, int(nrep), _ptr(st, _i32p)))
+        return st
+
+    def get(self):
+        nd = self.ndata
+        th = np.zeros((nd, BP))
+        ll = np.zeros(nd)
+        beta = np.zeros((nd, BQ))
+        st = np.zeros(nd, np.int32)
+        it = np.zeros(nd, np.int32)
+        ne = np.zeros(nd, np.int32)
+        c = self.ctx
+        c._check(self._L.svh_batch_get(c._h, _ptr(th), _ptr(ll), _ptr(beta), _ptr(st, _i32p),
+                                       _ptr(it, _i32p), _ptr(ne, _i32p)))
+        return th, ll, beta, st, it, ne
+
+    def set_reorder(self, i, rebuilt):
+        c = self.ctx
+        c._check(self._L.svh_batch_set_reorder(c._h, int(i), int(bool(rebuilt))))
+
+    def eval(self, theta):
+        nd = self.ndata
+        th = _f64(theta)
+        ll = np.zeros(nd)
+        g = np.zeros((nd, BP))
+        F = np.zeros((nd, BP * BP))
+        beta = np.zeros((nd, BQ))
+        jr = ctypes.c_int32(0)
+        c = self.ctx
+        c._check(self._L.svh_batch_eval(c._h, _ptr(th), _ptr(ll), _ptr(g), _ptr(F), _ptr(beta),
+                                        ctypes.byref(jr)))
+        return ll, g, F, beta, jr.value
 
 
 def context(device: int = 0) -> Context:

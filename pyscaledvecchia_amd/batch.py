@@ -1,0 +1,201 @@
+"""Fit many independent datasets at once.
+
+Each dataset is fitted exactly as `ScaledVecchiaGP.fit` fits it (reference
+gp.py:138-239 with optimize.py:11-73): same standardisation, subsample,
+starting values, reorder schedule and Fisher-scoring rules.  The difference is
+where the work runs: every objective evaluation of every dataset that is
+evaluating goes through one launch per kernel (uneven n, d, nu and trend are
+mixed in one launch), the per-dataset Fisher-scoring state machine runs on the
+GPU, and one step (evaluation + state update) is a captured hipGraph that the
+host replays.  The host only steps in for the reorder hook (maximin order and
+neighbours of the datasets whose ranges moved) and the final b-step.
+
+Limits of the batched kernels: d <= 10, trend columns <= 11, m_est <= 31.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .gp import ScaledVecchiaGP
+from .likelihood import _chunk_ids
+from .ordering import _nn_groups
+
+__all__ = ["fit_batch", "BatchedFit"]
+
+_NU_CODE = {0.5: 0, 1.5: 1, 2.5: 2}
+_DONE, _FAILED, _REORDER = 3, 4, 2
+
+
+class BatchedFit:
+    """Batched fitting driver.
+
+    models: ScaledVecchiaGP instances (one per dataset) holding the settings
+    (m_est, n_est, nu, trend, nugget, var_penalty, lambda_max, max_iter, tol,
+    random_state).  All models must share var_penalty, max_iter, tol and
+    lambda_max (they parameterise the one device state machine).
+    """
+
+    def __init__(self, models, use_graph=True, device=0):
+        self.models = list(models)
+        self.use_graph = use_graph
+        self.device = device
+        m0 = self.models[0]
+        for g in self.models:
+            if (g.var_penalty, g.max_iter, g.tol, g.lambda_max) != \
+                    (m0.var_penalty, m0.max_iter, m0.tol, m0.lambda_max):
+                raise ValueError("batched models must share var_penalty, max_iter, tol, "
+                                 "lambda_max")
+        self.stats = {}
+
+    def fit(self, Xs, ys):
+        t_start = time.perf_counter()
+        ctx = _lib.context(self.device)
+        models = self.models
+        nd = len(models)
+        if len(Xs) != nd or len(ys) != nd:
+            raise ValueError("one (X, y) per model")
+        preps = []
+        t0 = time.perf_counter()
+        for g, X, y in zip(models, Xs, ys):
+            p = g._prepare(X, y)
+            if p["d"] > 10 or g._design(np.zeros((1, p["d"]))).shape[1] > 11 or p["m"] > 31:
+                raise ValueError("batched fits need d <= 10 and m_est <= 31")
+            preps.append(p)
+        mmax = max(p["m"] for p in preps)
+        structs = self._structures(ctx, models, preps, [p["theta"] for p in preps], mmax)
+        t_build = time.perf_counter() - t0
+        # concatenate
+        d = np.array([p["d"] for p in preps], np.int32)
+        q = np.array([s["Zo"].shape[1] for s in structs], np.int32)
+        nu = np.array([_NU_CODE[g.nu] for g in models], np.int32)
+        n = np.array([len(s["yo"]) for s in structs], np.int64)
+        X = np.concatenate([s["Xo"].ravel() for s in structs])
+        y = np.concatenate([s["yo"] for s in structs])
+        Z = np.concatenate([s["Zo"].ravel() for s in structs]) if q.sum() else np.zeros(0)
+        nbr = np.concatenate([s["nbr"] for s in structs])
+        kcnt = np.concatenate([s["kcnt"] for s in structs])
+        chunk = np.concatenate([s["chunk"] for s in structs])
+        nchunk = np.array([s["nchunk"] for s in structs], np.int32)
+        lvt = np.array([p["lvt"] for p in preps])
+        fixed = np.array([g.nugget is not None for g in models], np.int32)
+        lnug = np.array([math.log(g.nugget) if g.nugget is not None else 0.0 for g in models])
+        bh = _lib.BatchHandle(ctx)
+        bh.setup(d, q, nu, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt, fixed, lnug,
+                 models[0].lambda_max)
+        theta0 = np.zeros((nd, _lib.BP))
+        for i, p in enumerate(preps):
+            theta0[i, :p["d"] + 2] = p["theta"]
+        last = [np.exp(p["theta"][1:1 + p["d"]]).copy() for p in preps]
+        bh.fit_start(theta0, models[0].max_iter, models[0].tol, models[0].var_penalty,
+                     self.use_graph)
+        steps, rebuilds, t_reorder = 0, [1] * nd, 0.0
+        t0 = time.perf_counter()
+        while True:
+            st = bh.fit_step(1)
+            steps += 1
+            if np.all((st == _DONE) | (st == _FAILED)):
+                break
+            pend = np.flatnonzero(st == _REORDER)
+            if pend.size:
+                tr = time.perf_counter()
+                th = bh.get()[0]
+                redo = []
+                for i in pend:
+                    g, p = models[i], preps[i]
+                    dd = p["d"]
+                    thi = g._clip_theta(th[i, :dd + 2].copy(), dd)
+                    r = np.exp(thi[1:1 + dd])
+                    if np.max(np.abs(np.log(r / last[i]))) < 1e-3:
+                        bh.set_reorder(i, False)
+                        continue
+                    redo.append(i)
+                    last[i] = r.copy()
+                if redo:
+                    new = self._structures(ctx, [models[i] for i in redo], [preps[i] for i in redo],
+                                           [th[i, :preps[i]["d"] + 2] for i in redo], mmax)
+                    for i, s in zip(redo, new):
+                        structs[i] = s
+                        bh.update(i, s["Xo"], s["yo"], s["Zo"], s["nbr"], s["kcnt"])
+                        bh.set_reorder(i, True)
+                        rebuilds[i] += 1
+                t_reorder += time.perf_counter() - tr
+            if steps > 100000:
+                raise RuntimeError("batched Fisher scoring did not terminate")
+        t_loop = time.perf_counter() - t0
+        th, ll, beta, st, iters, nev = bh.get()
+        for i, (g, p) in enumerate(zip(models, preps)):
+            if st[i] == _FAILED:
+                raise np.linalg.LinAlgError(f"dataset {i}: block Cholesky failed even with jitter")
+            dd, qq = p["d"], int(q[i])
+            g.n_evals_ = int(nev[i])
+            g.n_rebuilds_ = rebuilds[i]
+            g.n_iter_ = int(iters[i])
+            g._finish(th[i, :dd + 2], float(ll[i]), beta[i, :qq].copy(), p["rng"], p["n"])
+        self.stats = dict(steps=steps, wall_s=time.perf_counter() - t_start,
+                          build_s=t_build, loop_s=t_loop, reorder_s=t_reorder,
+                          evals=int(nev.sum()), rebuilds=int(sum(rebuilds)))
+        return models
+
+    @staticmethod
+    def _structures(ctx, models, preps, thetas, mmax):
+        """Structures of several datasets, their orderings and neighbour sets
+        computed together (svh_ordering_batch)."""
+        by_m = {}
+        for k, p in enumerate(preps):
+            by_m.setdefault(p["m"], []).append(k)
+        out = [None] * len(preps)
+        for m, ks in by_m.items():
+            Xs = [preps[k]["Xe"] / np.exp(np.asarray(thetas[k])[1:1 + preps[k]["d"]]) for k in ks]
+            res = ctx.ordering_batch(Xs, m)
+            for k, (order, nn) in zip(ks, res):
+                out[k] = BatchedFit._assemble(preps[k], order, nn, mmax)
+        return out
+
+    @staticmethod
+    def _assemble(p, order, nn, mmax):
+        Xe, ye, Ze, m = p["Xe"], p["ye"], p["Ze"], p["m"]
+        groups = _nn_groups(nn, m)
+        n = len(order)
+        kcnt = np.minimum(np.arange(n), m).astype(np.int32)
+        nbr = np.full((n, mmax), -1, np.int64)
+        nbr[:, :m] = nn[:, 1:m + 1]
+        chunk, nchunk = _chunk_ids(groups, p["d"] + 2)
+        return dict(Xo=np.ascontiguousarray(Xe[order]), yo=np.ascontiguousarray(ye[order]),
+                    Zo=np.ascontiguousarray(Ze[order]), nbr=nbr, kcnt=kcnt,
+                    chunk=np.asarray(chunk, np.int32), nchunk=nchunk)
+
+    @staticmethod
+    def _structure(g, p, theta, mmax):
+        """Ordered inputs, per-row neighbour lists and jitter-chunk ids of one
+        dataset at `theta` (gp.py:182-191 + likelihood.py:111-114)."""
+        Xe, ye, Ze, m = p["Xe"], p["ye"], p["Ze"], p["m"]
+        order, nn = g._order_structure(Xe, np.asarray(theta, float), m)
+        groups = _nn_groups(nn, m)
+        n = len(order)
+        kcnt = np.minimum(np.arange(n), m).astype(np.int32)
+        nbr = np.full((n, mmax), -1, np.int64)
+        nbr[:, :m] = nn[:, 1:m + 1]
+        chunk, nchunk = _chunk_ids(groups, p["d"] + 2)
+        return dict(Xo=np.ascontiguousarray(Xe[order]), yo=np.ascontiguousarray(ye[order]),
+                    Zo=np.ascontiguousarray(Ze[order]), nbr=nbr, kcnt=kcnt,
+                    chunk=np.asarray(chunk, np.int32), nchunk=nchunk)
+
+
+def fit_batch(Xs, ys, nus=None, use_graph=True, **kw):
+    """Fit one ScaledVecchiaGP per (X, y) pair in one batched run.
+
+    `nus` gives each dataset's smoothness (default: kw.get("nu", 2.5));
+    other keyword arguments are ScaledVecchiaGP settings shared by all.
+    Returns the fitted models (usable for predict / predict_joint).
+    """
+    nd = len(Xs)
+    nus = [kw.pop("nu", 2.5)] * nd if nus is None else list(nus)
+    models = [ScaledVecchiaGP(nu=nu, **kw) for nu in nus]
+    bf = BatchedFit(models, use_graph=use_graph)
+    bf.fit(Xs, ys)
+    return models, bf.stats

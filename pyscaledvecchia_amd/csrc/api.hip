@@ -20,6 +20,26 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
                            const double* theta, double nu, const double* dresid, int mode,
                            double* dout_a, double* dmean, double* dvar, int* jitter_power_max);
 
+void batch_release(svh_ctx* ctx);
+void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t* d,
+                           const int64_t* first, const double* X, int m, int64_t* order,
+                           int64_t* nn);
+void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t* q,
+                        const int32_t* nu_code, const int64_t* n, const double* X, const double* y,
+                        const double* Z, const int64_t* nbr, const int32_t* kcnt,
+                        const int32_t* chunk, const int32_t* nchunk, int mmax, const double* lvt,
+                        const int32_t* fixed_nug, const double* log_nug, double lambda_max);
+void batch_update_device(svh_ctx* ctx, int i, const double* X, const double* y, const double* Z,
+                         const int64_t* nbr, const int32_t* kcnt);
+void batch_fit_start_device(svh_ctx* ctx, const double* theta0, int max_iter, double tol,
+                            double var_penalty, int use_graph);
+void batch_fit_step_device(svh_ctx* ctx, int nrep, int32_t* status);
+void batch_get_device(svh_ctx* ctx, double* theta, double* ll, double* beta, int32_t* status,
+                      int32_t* iters, int32_t* nevals);
+void batch_set_reorder_device(svh_ctx* ctx, int i, int rebuilt);
+void batch_eval_device(svh_ctx* ctx, const double* theta, double* ll, double* grad,
+                       double* fisher, double* beta, int32_t* jitter_rounds);
+
 namespace {
 
 template <class F>
@@ -86,6 +106,7 @@ void svh_destroy(svh_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  batch_release(ctx);
   ctx->timers.release();
   ctx->arena.release();
   ctx->stage.release();
@@ -240,6 +261,73 @@ int svh_block_columns(svh_ctx* ctx, const double* Xall, int64_t nall, int32_t d,
                           nullptr, 0, da, nullptr, nullptr, nullptr);
     download(ctx, a, da, (size_t)nblk * K * sizeof(double));
     sync(ctx);
+  });
+}
+
+int svh_ordering_batch(svh_ctx* ctx, int32_t ndata, const int64_t* n, const int32_t* d,
+                       const int64_t* first, const double* X, int32_t m, int64_t* order,
+                       int64_t* nn) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(ndata >= 1 && n && d && first && X && order && nn && m >= 0,
+                "ordering_batch: bad argument");
+    ordering_batch_device(ctx, ndata, n, d, first, X, m, order, nn);
+  });
+}
+
+int svh_batch_setup(svh_ctx* ctx, int32_t ndata, const int32_t* d, const int32_t* q,
+                    const int32_t* nu_code, const int64_t* n, const double* X, const double* y,
+                    const double* Z, const int64_t* nbr, const int32_t* kcnt,
+                    const int32_t* chunk, const int32_t* nchunk, int32_t mmax, const double* lvt,
+                    const int32_t* fixed_nug, const double* log_nug, double lambda_max) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(d && q && nu_code && n && X && y && nbr && kcnt && chunk && nchunk && lvt &&
+                    fixed_nug && log_nug,
+                "batch_setup: null pointer");
+    batch_setup_device(ctx, ndata, d, q, nu_code, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt,
+                       fixed_nug, log_nug, lambda_max);
+  });
+}
+
+int svh_batch_update(svh_ctx* ctx, int32_t i, const double* X, const double* y, const double* Z,
+                     const int64_t* nbr, const int32_t* kcnt) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(X && y && nbr && kcnt, "batch_update: null pointer");
+    batch_update_device(ctx, i, X, y, Z, nbr, kcnt);
+  });
+}
+
+int svh_batch_fit_start(svh_ctx* ctx, const double* theta0, int32_t max_iter, double tol,
+                        double var_penalty, int32_t use_graph) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(theta0, "batch_fit_start: null pointer");
+    batch_fit_start_device(ctx, theta0, max_iter, tol, var_penalty, use_graph);
+  });
+}
+
+int svh_batch_fit_step(svh_ctx* ctx, int32_t nrep, int32_t* status) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(status && nrep >= 1, "batch_fit_step: bad argument");
+    batch_fit_step_device(ctx, nrep, status);
+  });
+}
+
+int svh_batch_get(svh_ctx* ctx, double* theta, double* ll, double* beta, int32_t* status,
+                  int32_t* iters, int32_t* nevals) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(theta && ll && beta && status && iters && nevals, "batch_get: null pointer");
+    batch_get_device(ctx, theta, ll, beta, status, iters, nevals);
+  });
+}
+
+int svh_batch_set_reorder(svh_ctx* ctx, int32_t i, int32_t rebuilt) {
+  return guarded(ctx, [&] { batch_set_reorder_device(ctx, i, rebuilt); });
+}
+
+int svh_batch_eval(svh_ctx* ctx, const double* theta, double* ll, double* grad, double* fisher,
+                   double* beta, int32_t* jitter_rounds) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(theta && ll && grad && fisher && beta, "batch_eval: null pointer");
+    batch_eval_device(ctx, theta, ll, grad, fisher, beta, jitter_rounds);
   });
 }
 

@@ -25,8 +25,12 @@
 #include "svh_internal.h"
 
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
+#include <cstring>
+#include <limits>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 const double kJitterPow10[10] = {0.0,  1e-12, 1e-11, 1e-10, 1e-09,
@@ -72,6 +76,7 @@ struct LLArgs {
   int64_t nblk;
   int d, q, mmax, P, need_grad, nu_code;
   double var, nug;
+  int rec_stride;  // doubles between consecutive block records
 };
 
 __device__ __forceinline__ void matern(int nu_code, double r, double& M, double& g) {
@@ -327,8 +332,15 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 }
 
 template <int NU>
-__device__ __forceinline__ void matern_t(double r, double& M, double& g) {
-  if constexpr (NU == 0) {
+__device__ __forceinline__ void matern_t(double r, double& M, double& g, int nu_rt = 2) {
+  if constexpr (NU < 0) {
+    if (nu_rt == 0)
+      matern_t<0>(r, M, g);
+    else if (nu_rt == 1)
+      matern_t<1>(r, M, g);
+    else
+      matern_t<2>(r, M, g);
+  } else if constexpr (NU == 0) {
     if (r > 0.0) {
       double e = exp(-r);
       M = e;
@@ -355,9 +367,12 @@ __host__ __device__ inline int split_lds_doubles(int d) {
   return 32 + 24 + 8 + 32 * 32 + 32 * d + 32 * 33;
 }
 
+// One block of the split-row kernel (body shared by the single-structure and
+// the batched kernels).  Accumulates this block's Fisher / GLS contributions
+// into the lane's task accumulators and writes the block record.
 template <int NU>
-__global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* lds,
+                                            double (&accF)[2], double (&accX)[3]) {
   const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
@@ -370,246 +385,253 @@ __global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
   const int nF = grad ? P * (P + 1) / 2 : 0;
   const int nT = grad ? P + P * q : 0;
   const int nX = q * q + q;
-  const int NC = grad ? 2 + q + 2 * P + P * q : 2 + q;
   const int dlo = h ? (d + 1) / 2 : 0, dhi = h ? d : (d + 1) / 2;   // this half's dims
-  double accF[2] = {0.0, 0.0};   // Fisher tasks lane, lane+64 (nF <= 78)
-  double accX[3] = {0.0, 0.0, 0.0};  // GLS tasks lane, +64, +128 (nX <= 132)
-
-  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x) {
-    const int K = a.kcnt[b] + 1;
-    const int ch = a.chunk[b];
-    const int lev = a.level[ch];
-    const bool act = r < K;
-    int pt = 0;
-    if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
-    __syncthreads();
-    for (int l = dlo; l < dhi; ++l)
-      Xs[r * d + l] = act ? a.X[(int64_t)pt * d + l] * a.inv_lam[l] : 0.0;
-    __syncthreads();
-    double xr[kSD];
+  const int K = a.kcnt[b] + 1;
+  const int ch = a.chunk[b];
+  const int lev = a.level[ch];
+  const bool act = r < K;
+  int pt = 0;
+  if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
+  __syncthreads();
+  for (int l = dlo; l < dhi; ++l)
+    Xs[r * d + l] = act ? a.X[(int64_t)pt * d + l] * a.inv_lam[l] : 0.0;
+  __syncthreads();
+  double xr[kSD];
 #pragma unroll
-    for (int l = 0; l < kSD; ++l) xr[l] = (l < d) ? Xs[r * d + l] : 0.0;
-    // ---- pairs: Sigma lower, g upper
+  for (int l = 0; l < kSD; ++l) xr[l] = (l < d) ? Xs[r * d + l] : 0.0;
+  // ---- pairs: Sigma lower, g upper
 #pragma unroll 1
-    for (int t = 9 * h; t < 9 * h + 9 && t <= 16; ++t) {
-      const int c = (r + t) & 31;
-      double Mv = 0.0, gg = 0.0;
-      if (t == 0) {
-        Mv = act ? a.var * (1.0 + a.nug) : 1.0;
-      } else if (act && c < K) {
-        double s2 = 0.0;
-#pragma unroll
-        for (int l = 0; l < kSD; ++l)
-          if (l < d) {
-            const double dd = xr[l] - Xs[c * d + l];
-            s2 = fma(dd, dd, s2);
-          }
-        matern_t<NU>(sqrt(s2), Mv, gg);
-        Mv *= a.var;
-      }
-      const int hi = r > c ? r : c, lo = r > c ? c : r;
-      Mt[hi * 33 + lo] = Mv;
-      if (t != 0) Mt[lo * 33 + hi] = gg;
-    }
-    __syncthreads();
-    double S[kSH];
-#pragma unroll
-    for (int cc = 0; cc < kSH; ++cc) {
-      const int c = kSH * h + cc;
-      S[cc] = (c <= r) ? Mt[r * 33 + c] : Mt[c * 33 + r];
-    }
-    // ---- chunk jitter (rare): Sigma += 10^p tr(Sigma)/K I on the real rows
-    double jit = 0.0;
-    if (lev > 0) {
-      __syncthreads();
-      if (h == 0) Cb[r] = act ? Mt[r * 33 + r] : 0.0;
-      __syncthreads();
-      double tr = 0.0;
-      for (int t = 0; t < K; ++t) tr += Cb[t];
-      jit = kJitDev[lev] * (tr / K);
-#pragma unroll
-      for (int cc = 0; cc < kSH; ++cc)
-        if (kSH * h + cc == r && act) S[cc] += jit;
-    }
-    // ---- Cholesky, column-scaled
-    bool fail = false;
-    double piv = 1.0;
-#pragma unroll
-    for (int k = 0; k < kSK; ++k) {
-      const int hk = k / kSH, kc = k % kSH;
-      __syncthreads();
-      if (h == hk) Cb[r] = S[kc];
-      __syncthreads();
-      const double pk = Cb[k];
-      fail |= (k < K) && !(pk > 0.0);
-      const double lp = Cb[r] * rcp_nr(pk);
-      if (r == k) piv = pk;
-      Lc[k * 32 + r] = lp;   // column k of L' (rows <= k hold junk, never read)
-      // trailing update of this lane's columns c > k (columns <= k hold junk)
-#pragma unroll
-      for (int cc = 0; cc < kSH; ++cc) {
-        if (k >= kSH && cc <= k - kSH) continue;   // no lane owns a live column
-        S[cc] = fma(-lp, Cb[kSH * h + cc], S[cc]);
-      }
-      // keep each step's update in place: without this the scheduler defers
-      // the FMAs across steps and holds every step's operands in registers
-#pragma unroll
-      for (int cc = 0; cc < kSH; ++cc) asm volatile("" : "+v"(S[cc]));
-    }
-    const double dinv = rsqrt_nr(piv);   // 1 / L_rr
-    if (r == K - 1 && h == 0) {
-      Misc[0] = dinv;
-      Misc[1] = piv;
-      Misc[2] = 0.5 * log(piv);
-    }
-    __syncthreads();
-    // ---- a = L^-T e_K:  L'^T at = e_K, a = at / L_KK  (both halves, same values)
-    double at = (r == K - 1) ? 1.0 : 0.0;
-#pragma unroll 4
-    for (int i = kSK - 1; i >= 0; --i) {
-      const double ai = __shfl(at, i, 32);
-      if (r < i) at = fma(-Lc[r * 32 + i], ai, at);
-    }
-    const double dK = Misc[0];
-    const double pK = Misc[1];
-    const double av = at * dK;
-    if (h == 0) Cb[r] = av;
-    __syncthreads();
-    // ---- right-hand sides
-    double R[kSC];
-#pragma unroll
-    for (int c = 0; c < kSC; ++c) R[c] = 0.0;
-    if (grad) {
-      // dS_{1+l} a for this half's dims, then half 1 gathers all of them
-      double accl[kSD];
-#pragma unroll
-      for (int l = 0; l < kSD; ++l) accl[l] = 0.0;
-#pragma unroll 2
-      for (int j = 0; j < kSK; ++j) {
-        const double gj = (j == r) ? 0.0 : ((j < r) ? Mt[j * 33 + r] : Mt[r * 33 + j]);
-        const double wj = gj * Cb[j];
-#pragma unroll
-        for (int l = 0; l < kSD; ++l)
-          if (l >= dlo && l < dhi) {
-            const double dd = xr[l] - Xs[j * d + l];
-            accl[l] = fma(wj * dd, dd, accl[l]);
-          }
-      }
-#pragma unroll
-      for (int l = 0; l < kSD; ++l) {
-        const double other = __shfl_xor(accl[l], 32, 64);
-        const double full = (l >= dlo && l < dhi) ? accl[l] : other;
-        if (l < d) R[1 + l] = h ? -a.var * full : R[1 + l];
-      }
-      if (h == 1) {
-        R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
-        R[kSC - 1] = a.var * a.nug * av;
-      }
-    }
-    if (h == 0 && act) {
-      R[0] = a.y[pt];
-#pragma unroll
-      for (int c = 0; c < kSQ; ++c)
-        if (c < q) R[1 + c] = a.Z[(int64_t)pt * q + c];
-    }
-    // ---- forward solve with the unit factor L', then scale by 1/L_rr
-    const int ncol = grad ? 2 : 1;   // halves that carry columns
-#pragma unroll 1
-    for (int j = 0; j < kSK - 1; ++j) {
-      __syncthreads();
-      if (r == j && h < ncol) {
-#pragma unroll
-        for (int c = 0; c < kSC; ++c) Bc[kSC * h + c] = R[c];
-      }
-      __syncthreads();
-      if (h < ncol) {
-        const double wj = (j < r) ? Lc[j * 32 + r] : 0.0;   // L'_rj
-#pragma unroll
-        for (int c = 0; c < kSC; ++c) R[c] = fma(-wj, Bc[kSC * h + c], R[c]);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kSC; ++c) R[c] *= dinv;
-    const bool ok = !fail;
-    if (fail && lane == 0) atomicOr(a.fail + ch, 1);
-    __syncthreads();
-    // ---- columns to LDS by parameter index: W_p (p < P), then yt, Zt_c
-    const int cyl = grad ? P : 0;
-    if (h == 1 && grad) {
-      Mt[0 * 33 + r] = R[0];
+  for (int t = 9 * h; t < 9 * h + 9 && t <= 16; ++t) {
+    const int c = (r + t) & 31;
+    double Mv = 0.0, gg = 0.0;
+    if (t == 0) {
+      Mv = act ? a.var * (1.0 + a.nug) : 1.0;
+    } else if (act && c < K) {
+      double s2 = 0.0;
 #pragma unroll
       for (int l = 0; l < kSD; ++l)
-        if (l < d) Mt[(1 + l) * 33 + r] = R[1 + l];
-      Mt[(d + 1) * 33 + r] = R[kSC - 1];
-    }
-    if (h == 0) {
-      Mt[cyl * 33 + r] = R[0];
-#pragma unroll
-      for (int c = 0; c < kSQ; ++c)
-        if (c < q) Mt[(cyl + 1 + c) * 33 + r] = R[1 + c];
-    }
-    __syncthreads();
-    const int km = K - 1;
-    double* rec = a.cache + b * NC;
-    for (int e = lane; e < 2 + q + (grad ? P : 0); e += kWave) {
-      double v;
-      if (e == 0)
-        v = Mt[cyl * 33 + km];
-      else if (e <= q)
-        v = Mt[(cyl + e) * 33 + km];
-      else if (e == q + 1)
-        v = Misc[2];
-      else
-        v = Mt[(e - q - 2) * 33 + km];
-      if (ok) rec[e] = v;
-    }
-    for (int e = lane; e < nF + nT; e += kWave) {
-      int c1, c2;
-      if (e < nF) {
-        pair_of(e, c1, c2);
-      } else if (e < nF + P) {
-        c1 = e - nF;
-        c2 = cyl;
-      } else {
-        const int rr = e - nF - P;
-        c1 = rr / q;
-        c2 = cyl + 1 + (rr - c1 * q);
-      }
-      double sdot = 0.0;
-      for (int i = 0; i < K; ++i) sdot = fma(Mt[c1 * 33 + i], Mt[c2 * 33 + i], sdot);
-      if (e < nF) {
-        const double f = sdot - 0.5 * Mt[c1 * 33 + km] * Mt[c2 * 33 + km];
-        if (ok) {
-          if (e < kWave)
-            accF[0] += f;
-          else
-            accF[1] += f;
+        if (l < d) {
+          const double dd = xr[l] - Xs[c * d + l];
+          s2 = fma(dd, dd, s2);
         }
-      } else if (e < nF + P) {
-        if (ok) rec[2 + q + P + (e - nF)] = sdot;
-      } else {
-        if (ok) rec[2 + q + 2 * P + (e - nF - P)] = sdot;
-      }
+      matern_t<NU>(sqrt(s2), Mv, gg, a.nu_code);
+      Mv *= a.var;
     }
-    for (int e = lane; e < nX; e += kWave) {
-      double v;
-      if (e < q * q) {
-        const int c1 = e / q, c2 = e - c1 * q;
-        v = Mt[(cyl + 1 + c1) * 33 + km] * Mt[(cyl + 1 + c2) * 33 + km];
-      } else {
-        const int c = e - q * q;
-        v = Mt[(cyl + 1 + c) * 33 + km] * Mt[cyl * 33 + km];
-      }
-      if (ok) {
-        if (e < kWave)
-          accX[0] += v;
-        else if (e < 2 * kWave)
-          accX[1] += v;
-        else
-          accX[2] += v;
-      }
+    const int hi = r > c ? r : c, lo = r > c ? c : r;
+    Mt[hi * 33 + lo] = Mv;
+    if (t != 0) Mt[lo * 33 + hi] = gg;
+  }
+  __syncthreads();
+  double S[kSH];
+#pragma unroll
+  for (int cc = 0; cc < kSH; ++cc) {
+    const int c = kSH * h + cc;
+    S[cc] = (c <= r) ? Mt[r * 33 + c] : Mt[c * 33 + r];
+  }
+  // ---- chunk jitter (rare): Sigma += 10^p tr(Sigma)/K I on the real rows
+  double jit = 0.0;
+  if (lev > 0) {
+    __syncthreads();
+    if (h == 0) Cb[r] = act ? Mt[r * 33 + r] : 0.0;
+    __syncthreads();
+    double tr = 0.0;
+    for (int t = 0; t < K; ++t) tr += Cb[t];
+    jit = kJitDev[lev] * (tr / K);
+#pragma unroll
+    for (int cc = 0; cc < kSH; ++cc)
+      if (kSH * h + cc == r && act) S[cc] += jit;
+  }
+  // ---- Cholesky, column-scaled
+  bool fail = false;
+  double piv = 1.0;
+#pragma unroll
+  for (int k = 0; k < kSK; ++k) {
+    const int hk = k / kSH, kc = k % kSH;
+    __syncthreads();
+    if (h == hk) Cb[r] = S[kc];
+    __syncthreads();
+    const double pk = Cb[k];
+    fail |= (k < K) && !(pk > 0.0);
+    const double lp = Cb[r] * rcp_nr(pk);
+    if (r == k) piv = pk;
+    Lc[k * 32 + r] = lp;   // column k of L' (rows <= k hold junk, never read)
+    // trailing update of this lane's columns c > k (columns <= k hold junk)
+#pragma unroll
+    for (int cc = 0; cc < kSH; ++cc) {
+      if (k >= kSH && cc <= k - kSH) continue;   // no lane owns a live column
+      S[cc] = fma(-lp, Cb[kSH * h + cc], S[cc]);
+    }
+    // keep each step's update in place: without this the scheduler defers
+    // the FMAs across steps and holds every step's operands in registers
+#pragma unroll
+    for (int cc = 0; cc < kSH; ++cc) asm volatile("" : "+v"(S[cc]));
+  }
+  const double dinv = rsqrt_nr(piv);   // 1 / L_rr
+  if (r == K - 1 && h == 0) {
+    Misc[0] = dinv;
+    Misc[1] = piv;
+    Misc[2] = 0.5 * log(piv);
+  }
+  __syncthreads();
+  // ---- a = L^-T e_K:  L'^T at = e_K, a = at / L_KK  (both halves, same values)
+  double at = (r == K - 1) ? 1.0 : 0.0;
+#pragma unroll 4
+  for (int i = kSK - 1; i >= 0; --i) {
+    const double ai = __shfl(at, i, 32);
+    if (r < i) at = fma(-Lc[r * 32 + i], ai, at);
+  }
+  const double dK = Misc[0];
+  const double pK = Misc[1];
+  const double av = at * dK;
+  if (h == 0) Cb[r] = av;
+  __syncthreads();
+  // ---- right-hand sides
+  double R[kSC];
+#pragma unroll
+  for (int c = 0; c < kSC; ++c) R[c] = 0.0;
+  if (grad) {
+    // dS_{1+l} a for this half's dims, then half 1 gathers all of them
+    double accl[kSD];
+#pragma unroll
+    for (int l = 0; l < kSD; ++l) accl[l] = 0.0;
+#pragma unroll 2
+    for (int j = 0; j < kSK; ++j) {
+      const double gj = (j == r) ? 0.0 : ((j < r) ? Mt[j * 33 + r] : Mt[r * 33 + j]);
+      const double wj = gj * Cb[j];
+#pragma unroll
+      for (int l = 0; l < kSD; ++l)
+        if (l >= dlo && l < dhi) {
+          const double dd = xr[l] - Xs[j * d + l];
+          accl[l] = fma(wj * dd, dd, accl[l]);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < kSD; ++l) {
+      const double other = __shfl_xor(accl[l], 32, 64);
+      const double full = (l >= dlo && l < dhi) ? accl[l] : other;
+      if (l < d) R[1 + l] = h ? -a.var * full : R[1 + l];
+    }
+    if (h == 1) {
+      R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
+      R[kSC - 1] = a.var * a.nug * av;
     }
   }
+  if (h == 0 && act) {
+    R[0] = a.y[pt];
+#pragma unroll
+    for (int c = 0; c < kSQ; ++c)
+      if (c < q) R[1 + c] = a.Z[(int64_t)pt * q + c];
+  }
+  // ---- forward solve with the unit factor L', then scale by 1/L_rr
+  const int ncol = grad ? 2 : 1;   // halves that carry columns
+#pragma unroll 1
+  for (int j = 0; j < kSK - 1; ++j) {
+    __syncthreads();
+    if (r == j && h < ncol) {
+#pragma unroll
+      for (int c = 0; c < kSC; ++c) Bc[kSC * h + c] = R[c];
+    }
+    __syncthreads();
+    if (h < ncol) {
+      const double wj = (j < r) ? Lc[j * 32 + r] : 0.0;   // L'_rj
+#pragma unroll
+      for (int c = 0; c < kSC; ++c) R[c] = fma(-wj, Bc[kSC * h + c], R[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kSC; ++c) R[c] *= dinv;
+  const bool ok = !fail;
+  if (fail && lane == 0) atomicOr(a.fail + ch, 1);
+  __syncthreads();
+  // ---- columns to LDS by parameter index: W_p (p < P), then yt, Zt_c
+  const int cyl = grad ? P : 0;
+  if (h == 1 && grad) {
+    Mt[0 * 33 + r] = R[0];
+#pragma unroll
+    for (int l = 0; l < kSD; ++l)
+      if (l < d) Mt[(1 + l) * 33 + r] = R[1 + l];
+    Mt[(d + 1) * 33 + r] = R[kSC - 1];
+  }
+  if (h == 0) {
+    Mt[cyl * 33 + r] = R[0];
+#pragma unroll
+    for (int c = 0; c < kSQ; ++c)
+      if (c < q) Mt[(cyl + 1 + c) * 33 + r] = R[1 + c];
+  }
+  __syncthreads();
+  const int km = K - 1;
+  double* rec = a.cache + b * a.rec_stride;
+  for (int e = lane; e < 2 + q + (grad ? P : 0); e += kWave) {
+    double v;
+    if (e == 0)
+      v = Mt[cyl * 33 + km];
+    else if (e <= q)
+      v = Mt[(cyl + e) * 33 + km];
+    else if (e == q + 1)
+      v = Misc[2];
+    else
+      v = Mt[(e - q - 2) * 33 + km];
+    if (ok) rec[e] = v;
+  }
+  for (int e = lane; e < nF + nT; e += kWave) {
+    int c1, c2;
+    if (e < nF) {
+      pair_of(e, c1, c2);
+    } else if (e < nF + P) {
+      c1 = e - nF;
+      c2 = cyl;
+    } else {
+      const int rr = e - nF - P;
+      c1 = rr / q;
+      c2 = cyl + 1 + (rr - c1 * q);
+    }
+    double sdot = 0.0;
+    for (int i = 0; i < K; ++i) sdot = fma(Mt[c1 * 33 + i], Mt[c2 * 33 + i], sdot);
+    if (e < nF) {
+      const double f = sdot - 0.5 * Mt[c1 * 33 + km] * Mt[c2 * 33 + km];
+      if (ok) {
+        if (e < kWave)
+          accF[0] += f;
+        else
+          accF[1] += f;
+      }
+    } else if (e < nF + P) {
+      if (ok) rec[2 + q + P + (e - nF)] = sdot;
+    } else {
+      if (ok) rec[2 + q + 2 * P + (e - nF - P)] = sdot;
+    }
+  }
+  for (int e = lane; e < nX; e += kWave) {
+    double v;
+    if (e < q * q) {
+      const int c1 = e / q, c2 = e - c1 * q;
+      v = Mt[(cyl + 1 + c1) * 33 + km] * Mt[(cyl + 1 + c2) * 33 + km];
+    } else {
+      const int c = e - q * q;
+      v = Mt[(cyl + 1 + c) * 33 + km] * Mt[cyl * 33 + km];
+    }
+    if (ok) {
+      if (e < kWave)
+        accX[0] += v;
+      else if (e < 2 * kWave)
+        accX[1] += v;
+      else
+        accX[2] += v;
+    }
+  }
+}
+
+template <int NU>
+__global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  const int P = a.P, q = a.q;
+  const bool grad = a.need_grad != 0;
+  const int nF = grad ? P * (P + 1) / 2 : 0;
+  const int nX = q * q + q;
+  double accF[2] = {0.0, 0.0};   // Fisher tasks lane, lane+64 (nF <= 78)
+  double accX[3] = {0.0, 0.0, 0.0};  // GLS tasks lane, +64, +128 (nX <= 132)
+  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x) split_block<NU>(a, b, lds, accF, accX);
   // this workgroup's partials (layout of ll_pass1: F (P*P), XtX (q*q), Zty (q))
   const int NACC = grad ? P * P + q * q + q : q * q + q;
   const int X0 = grad ? P * P : 0;
@@ -652,17 +674,20 @@ __global__ __launch_bounds__(256) void ll_sum_partials(const double* __restrict_
   if (threadIdx.x == 0) tot[e] = red[0];
 }
 
-__global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ tot, int P, int q,
-                                                     int need_grad, double* __restrict__ beta,
-                                                     double* __restrict__ out) {
+// Fisher to `out`, beta = solve(XtX + 1e-12 I, Zty) by Gaussian elimination
+// with partial pivoting, parallel over the trailing submatrix in LDS.
+// Fisher entries are stored with row stride `fst` (P for one structure).
+__device__ void ll_solve_beta_body(const double* __restrict__ tot, int P, int q, int need_grad,
+                                   double* __restrict__ beta, double* __restrict__ out, int Pout,
+                                   int fst) {
   __shared__ double A[32][34];
   __shared__ int piv_s;
   const int X0 = need_grad ? P * P : 0, Y0 = X0 + q * q;
   const int t = threadIdx.x;
   if (need_grad)
-    for (int e = t; e < P * P; e += 256) out[1 + P + e] = tot[e];
+    for (int e = t; e < P * P; e += blockDim.x) out[1 + Pout + e] = tot[e];
   if (q == 0) return;
-  for (int e = t; e < q * (q + 1); e += 256) {
+  for (int e = t; e < q * (q + 1); e += blockDim.x) {
     int i = e / (q + 1), j = e - i * (q + 1);
     A[i][j] = (j < q) ? tot[X0 + i * q + j] + (i == j ? 1e-12 : 0.0) : tot[Y0 + i];
   }
@@ -681,20 +706,20 @@ __global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ 
     __syncthreads();
     const int p = piv_s;
     if (p != k)
-      for (int j = t; j <= q; j += 256) {
+      for (int j = t; j <= q; j += blockDim.x) {
         double tmp = A[k][j];
         A[k][j] = A[p][j];
         A[p][j] = tmp;
       }
     __syncthreads();
     const int rem = q - 1 - k, w = q + 1 - k;
-    for (int e = t; e < rem * w; e += 256) {
+    for (int e = t; e < rem * w; e += blockDim.x) {
       int i = k + 1 + e / w, j = k + (e % w);
       if (j == k) continue;
       A[i][j] -= (A[i][k] / A[k][k]) * A[k][j];
     }
     __syncthreads();
-    for (int i = k + 1 + t; i < q; i += 256) A[i][k] = 0.0;
+    for (int i = k + 1 + t; i < q; i += blockDim.x) A[i][k] = 0.0;
     __syncthreads();
   }
   if (t == 0) {
@@ -705,9 +730,16 @@ __global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ 
     }
     for (int i = 0; i < q; ++i) {
       beta[i] = A[i][33];
-      out[1 + P + P * P + i] = A[i][33];
+      out[1 + Pout + Pout * Pout + i] = A[i][33];
     }
   }
+  (void)fst;
+}
+
+__global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ tot, int P, int q,
+                                                     int need_grad, double* __restrict__ beta,
+                                                     double* __restrict__ out) {
+  ll_solve_beta_body(tot, P, q, need_grad, beta, out, P, P);
 }
 
 // ---------------------------------------------------------------- pass 2
@@ -784,6 +816,461 @@ __global__ void ll_escalate(int* __restrict__ level, int* __restrict__ fail, int
 __global__ void fill_int(int* p, int v, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ======================================================================
+// Batched fits: many independent datasets, every conditioning block of every
+// active dataset in one launch per kernel, and a per-dataset Fisher-scoring
+// state machine on the device (restates optimize.py:11-73 per dataset).
+// ======================================================================
+constexpr int kBP = 12;     // max parameters per dataset (d <= 10)
+constexpr int kBQ = 11;     // max trend columns
+constexpr int kBNACC = kBP * kBP + kBQ * kBQ + kBQ;
+constexpr int kBNC = 2 + kBQ + 2 * kBP + kBP * kBQ;   // record stride
+constexpr int kBNV = 1 + kBP;
+constexpr int kBOUT = 1 + kBP + kBP * kBP + kBQ + 1;  // ll, grad, F, beta, nfail
+
+struct DSDesc {
+  int d, q, P, nu_code, mmax, nchunk, fixed_nug, pad;
+  int64_t xoff, yoff, zoff, blk0, nblk, chunk0;
+  double lvt;        // log variance target of the sigma^2 penalty
+  double log_nug;    // fixed nugget (log) when fixed_nug
+  double lam_hi;     // log(lambda_max) (or +inf)
+};
+
+enum FSStatus : int {
+  FS_EVAL_INIT = 0,   // evaluate at theta (first objective call)
+  FS_SEARCH = 1,      // evaluate the trial point of the line search
+  FS_REORDER = 2,     // host: reorder hook due
+  FS_DONE = 3,
+  FS_FAILED = 4,      // Cholesky failed at the start or after a reorder
+  FS_REEVAL = 5,      // evaluate at theta after a rebuild
+  FS_STEP = 6,        // no evaluation: compute the next step (reorder skipped)
+};
+
+struct FSState {
+  double theta[kBP], trial[kBP], step[kBP], g[kBP], M[kBP * kBP], beta[kBQ];
+  double ll, alpha;
+  int status, k, next_reorder, halv, retry, nevals, max_iter, pad;
+  double tol;
+};
+
+__device__ __forceinline__ bool fs_evaluates(int st) {
+  return st == FS_EVAL_INIT || st == FS_SEARCH || st == FS_REEVAL;
+}
+
+// theta -> clipped evaluation point -> (var, nug, 1/lambda) per dataset
+__global__ void fs_params(const DSDesc* __restrict__ ds, const FSState* __restrict__ fs,
+                          double* __restrict__ par, double* __restrict__ th_eval, int ndata) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndata) return;
+  const DSDesc D = ds[i];
+  const FSState& F = fs[i];
+  const double* src = (F.status == FS_SEARCH) ? F.trial : F.theta;
+  double th[kBP];
+  for (int p = 0; p < D.P; ++p) th[p] = src[p];
+  for (int l = 0; l < D.d; ++l) th[1 + l] = fmin(fmax(th[1 + l], -18.420680743952367), D.lam_hi);
+  th[D.P - 1] = fmin(fmax(th[D.P - 1], -27.631021115928547), 6.907755278982137);
+  if (D.fixed_nug) th[D.P - 1] = D.log_nug;
+  double* pr = par + i * 16;
+  pr[0] = exp(th[0]);
+  pr[1] = exp(th[D.P - 1]);
+  for (int l = 0; l < D.d; ++l) pr[2 + l] = 1.0 / exp(th[1 + l]);
+  for (int p = 0; p < D.P; ++p) th_eval[i * kBP + p] = th[p];
+}
+
+// chunk levels back to 0 for datasets that are not retrying a jittered point
+__global__ void fs_reset_chunks(const int* __restrict__ chunk_ds, const FSState* __restrict__ fs,
+                                int* __restrict__ level, int* __restrict__ fail, int nchunk) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunk) return;
+  if (fs[chunk_ds[c]].retry == 0) level[c] = 0;
+  fail[c] = 0;
+}
+
+struct BatchArgs {
+  const DSDesc* ds;
+  const FSState* fs;
+  const double* par;
+  const double *X, *y, *Z;
+  const int *rows, *nbr, *kcnt, *chunk;
+  const int* level;
+  int* fail;
+  double* cache;
+  double* partial;   // (ndata, GX, kBNACC)
+  int mmax_stride, GX;
+};
+
+__global__ __launch_bounds__(kWave) void ll_pass1_batch(BatchArgs B) {
+  extern __shared__ double lds[];
+  const int i = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+  if (!fs_evaluates(B.fs[i].status)) return;
+  const DSDesc D = B.ds[i];
+  if (x >= D.nblk) return;   // small datasets leave spare workgroups idle
+  LLArgs a;
+  a.X = B.X + D.xoff;
+  a.y = B.y + D.yoff;
+  a.Z = B.Z + D.zoff;
+  a.rows = B.rows;
+  a.nbr = B.nbr;
+  a.kcnt = B.kcnt;
+  a.chunk = B.chunk;
+  a.level = B.level;
+  a.fail = B.fail;
+  a.inv_lam = B.par + i * 16 + 2;
+  a.cache = B.cache;
+  a.partial = nullptr;
+  a.nblk = D.blk0 + D.nblk;
+  a.d = D.d;
+  a.q = D.q;
+  a.mmax = B.mmax_stride;
+  a.P = D.P;
+  a.need_grad = 1;
+  a.nu_code = D.nu_code;
+  a.var = B.par[i * 16 + 0];
+  a.nug = B.par[i * 16 + 1];
+  a.rec_stride = kBNC;
+  double accF[2] = {0.0, 0.0};
+  double accX[3] = {0.0, 0.0, 0.0};
+  for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX) split_block<-1>(a, b, lds, accF, accX);
+  const int P = D.P, q = D.q;
+  const int nF = P * (P + 1) / 2, nX = q * q + q;
+  double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
+  for (int e = lane; e < nF; e += kWave) {
+    int c1, c2;
+    pair_of(e, c1, c2);
+    const double f = (e < kWave) ? accF[0] : accF[1];
+    part[c1 * P + c2] = f;
+    part[c2 * P + c1] = f;
+  }
+  for (int e = lane; e < nX; e += kWave)
+    part[P * P + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
+}
+
+// (entry, dataset) workgroups: fixed-order sums of the pass-1 partials
+__global__ __launch_bounds__(256) void ll_sum_batch(const DSDesc* __restrict__ ds,
+                                                    const FSState* __restrict__ fs,
+                                                    const double* __restrict__ partial, int GX,
+                                                    double* __restrict__ tot) {
+  __shared__ double red[256];
+  const int e = blockIdx.x, i = blockIdx.y;
+  if (!fs_evaluates(fs[i].status)) return;
+  const DSDesc D = ds[i];
+  if (e >= D.P * D.P + D.q * D.q + D.q) return;
+  const int gx = D.nblk < GX ? (int)D.nblk : GX;
+  double s = 0.0;
+  for (int x = threadIdx.x; x < gx; x += 256)
+    s += partial[((int64_t)i * GX + x) * kBNACC + e];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tot[(int64_t)i * kBNACC + e] = red[0];
+}
+
+__global__ __launch_bounds__(256) void ll_beta_batch(const DSDesc* __restrict__ ds,
+                                                     const FSState* __restrict__ fs,
+                                                     const double* __restrict__ tot,
+                                                     double* __restrict__ beta,
+                                                     double* __restrict__ out) {
+  const int i = blockIdx.x;
+  if (!fs_evaluates(fs[i].status)) return;
+  const DSDesc D = ds[i];
+  ll_solve_beta_body(tot + (int64_t)i * kBNACC, D.P, D.q, 1, beta + i * kBQ, out + i * kBOUT,
+                     D.P, kBP);
+}
+
+__global__ __launch_bounds__(256) void ll_pass2_batch(const DSDesc* __restrict__ ds,
+                                                      const FSState* __restrict__ fs,
+                                                      const double* __restrict__ cache,
+                                                      const double* __restrict__ beta, int G2X,
+                                                      double* __restrict__ partial2) {
+  __shared__ double red[256];
+  const int i = blockIdx.y;
+  if (!fs_evaluates(fs[i].status)) return;
+  const DSDesc D = ds[i];
+  const int P = D.P, q = D.q;
+  double bet[kBQ];
+  for (int c = 0; c < kBQ; ++c) bet[c] = c < q ? beta[i * kBQ + c] : 0.0;
+  double accv[kBNV];
+  for (int e = 0; e < kBNV; ++e) accv[e] = 0.0;
+  for (int64_t b = D.blk0 + blockIdx.x * 256 + threadIdx.x; b < D.blk0 + D.nblk;
+       b += (int64_t)G2X * 256) {
+    const double* rec = cache + b * kBNC;
+    double zK = rec[0];
+    for (int c = 0; c < q; ++c) zK -= rec[1 + c] * bet[c];
+    accv[0] += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+    const double* wK = rec + 2 + q;
+    const double* sv = wK + P;
+    const double* Tm = sv + P;
+    for (int p = 0; p < P; ++p) {
+      double wz = sv[p];
+      for (int c = 0; c < q; ++c) wz -= Tm[p * q + c] * bet[c];
+      accv[1 + p] += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
+    }
+  }
+  for (int e = 0; e < kBNV; ++e) {
+    red[threadIdx.x] = accv[e];
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial2[((int64_t)i * G2X + blockIdx.x) * kBNV + e] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void ll_reduce2_batch(const DSDesc* __restrict__ ds,
+                                                        const FSState* __restrict__ fs,
+                                                        const double* __restrict__ partial2,
+                                                        int G2X, const int* __restrict__ fail,
+                                                        double* __restrict__ out) {
+  __shared__ double red[256];
+  const int e = blockIdx.x, i = blockIdx.y;
+  if (!fs_evaluates(fs[i].status)) return;
+  const DSDesc D = ds[i];
+  if (e > 1 + D.P) return;
+  double s = 0.0;
+  if (e <= D.P) {
+    for (int g = threadIdx.x; g < G2X; g += 256) s += partial2[((int64_t)i * G2X + g) * kBNV + e];
+  } else {
+    for (int c = threadIdx.x; c < D.nchunk; c += 256) s += fail[D.chunk0 + c] ? 1.0 : 0.0;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)i * kBOUT + (e <= D.P ? e : kBOUT - 1)] = red[0];
+}
+
+// ---- small dense solves for the Fisher step (P <= 12), one thread
+__device__ bool lu_solve(double* A, double* b, int n) {
+  // Gaussian elimination with partial pivoting; false on an exactly zero pivot
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + k]) > best) {
+        best = fabs(A[i * n + k]);
+        p = i;
+      }
+    if (best == 0.0) return false;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) {
+        double t = A[k * n + j];
+        A[k * n + j] = A[p * n + j];
+        A[p * n + j] = t;
+      }
+      double t = b[k];
+      b[k] = b[p];
+      b[p] = t;
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+  return true;
+}
+
+// Fisher-scoring direction, ridged until it is an ascent direction
+// (optimize.py:28-45), capped at norm 3 (optimize.py:47-49); sets the trial.
+__device__ void fs_begin_search(FSState& F, int P) {
+  double A[kBP * kBP], x[kBP];
+  double dg[kBP];
+  for (int p = 0; p < P; ++p) dg[p] = fmax(F.M[p * P + p], 1e-8);
+  double ridge = 0.0;
+  bool have = false;
+  for (int t = 0; t < 12 && !have; ++t) {
+    for (int e = 0; e < P * P; ++e) A[e] = F.M[e];
+    for (int p = 0; p < P; ++p) {
+      A[p * P + p] += ridge * dg[p];
+      x[p] = F.g[p];
+    }
+    if (lu_solve(A, x, P)) {
+      double dot = 0.0;
+      bool fin = true;
+      for (int p = 0; p < P; ++p) {
+        dot += x[p] * F.g[p];
+        fin = fin && isfinite(x[p]);
+      }
+      if (dot > 0.0 && fin) {
+        have = true;
+        break;
+      }
+    }
+    ridge = fmax(1e-6, ridge * 10.0);
+  }
+  if (!have) {
+    double nrm = 0.0;
+    for (int p = 0; p < P; ++p) nrm += F.g[p] * F.g[p];
+    nrm = fmax(sqrt(nrm), 1e-12);
+    for (int p = 0; p < P; ++p) x[p] = F.g[p] / nrm;
+  }
+  double nrm = 0.0;
+  for (int p = 0; p < P; ++p) nrm += x[p] * x[p];
+  nrm = sqrt(nrm);
+  const double sc = nrm > 3.0 ? 3.0 / nrm : 1.0;
+  for (int p = 0; p < P; ++p) {
+    F.step[p] = x[p] * sc;
+    F.trial[p] = F.theta[p] + F.step[p];
+  }
+  F.alpha = 1.0;
+  F.halv = 0;
+  F.status = FS_SEARCH;
+}
+
+__device__ void fs_begin_iteration(FSState& F, int P) {
+  if (F.k > F.max_iter) {
+    F.status = FS_DONE;
+    return;
+  }
+  if (F.k == F.next_reorder) {
+    F.next_reorder *= 2;
+    F.status = FS_REORDER;
+    return;
+  }
+  fs_begin_search(F, P);
+}
+
+// One workgroup per dataset: escalate failed chunks, then advance the state.
+__global__ __launch_bounds__(64) void fs_update(const DSDesc* __restrict__ ds,
+                                                FSState* __restrict__ fs,
+                                                const double* __restrict__ out,
+                                                const double* __restrict__ beta,
+                                                int* __restrict__ level, int* __restrict__ fail,
+                                                double var_penalty, int* __restrict__ status_out) {
+  const int i = blockIdx.x;
+  FSState& F = fs[i];
+  const DSDesc D = ds[i];
+  const int P = D.P;
+  __shared__ int st0;
+  if (threadIdx.x == 0) st0 = F.status;
+  __syncthreads();
+  const int st = st0;
+  if (st == FS_STEP) {
+    if (threadIdx.x == 0) {
+      fs_begin_search(F, P);
+      status_out[i] = F.status;
+    }
+    return;
+  }
+  if (!fs_evaluates(st)) {
+    if (threadIdx.x == 0) status_out[i] = st;
+    return;
+  }
+  const double* o = out + (int64_t)i * kBOUT;
+  const int nfail = (int)o[kBOUT - 1];
+  if (nfail > 0) {
+    // jitter ladder: failed chunks go one power up and the point is re-evaluated
+    __shared__ int exhausted;
+    if (threadIdx.x == 0) exhausted = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < D.nchunk; c += 64) {
+      const int gc = D.chunk0 + c;
+      if (fail[gc]) {
+        if (level[gc] >= 9) atomicOr(&exhausted, 1);
+        level[gc] += 1;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (!exhausted) {
+        F.retry = 1;
+      } else {
+        // LinAlgError: fatal for the first evaluation / a re-evaluation,
+        // halves the step inside the line search (optimize.py:54-58)
+        F.retry = 0;
+        if (st == FS_SEARCH) {
+          F.alpha *= 0.5;
+          F.halv += 1;
+          if (F.halv >= 20) {
+            F.status = FS_DONE;
+          } else {
+            for (int p = 0; p < P; ++p) F.trial[p] = F.theta[p] + F.alpha * F.step[p];
+          }
+        } else {
+          F.status = FS_FAILED;
+        }
+      }
+      status_out[i] = F.status;
+    }
+    return;
+  }
+  if (threadIdx.x != 0) return;
+  F.retry = 0;
+  F.nevals += 1;
+  // objective value at the evaluated point: penalty and fixed-nugget freeze
+  const double* th = (st == FS_SEARCH) ? F.trial : F.theta;
+  double thc0 = th[0];
+  const double LOG2PI = 1.8378770664093453;
+  double ll = -0.5 * (double)D.nblk * LOG2PI + o[0];
+  double g[kBP], M[kBP * kBP];
+  for (int p = 0; p < P; ++p) g[p] = o[1 + p];
+  for (int e = 0; e < P * P; ++e) M[e] = o[1 + P + e];
+  if (var_penalty > 0.0) {
+    const double ex = fmax(0.0, thc0 - D.lvt);
+    ll -= 0.5 * var_penalty * ex * ex;
+    g[0] -= var_penalty * ex;
+    if (ex > 0.0) M[0] += var_penalty;
+  }
+  if (D.fixed_nug) {
+    g[P - 1] = 0.0;
+    for (int p = 0; p < P; ++p) {
+      M[(P - 1) * P + p] = 0.0;
+      M[p * P + (P - 1)] = 0.0;
+    }
+    M[P * P - 1] = 1.0;
+  }
+  if (st == FS_EVAL_INIT || st == FS_REEVAL) {
+    F.ll = ll;
+    for (int p = 0; p < P; ++p) F.g[p] = g[p];
+    for (int e = 0; e < P * P; ++e) F.M[e] = M[e];
+    for (int c = 0; c < D.q; ++c) F.beta[c] = beta[i * kBQ + c];
+    if (st == FS_EVAL_INIT) {
+      F.k = 1;
+      F.next_reorder = 2;
+      fs_begin_iteration(F, P);
+    } else {
+      fs_begin_search(F, P);
+    }
+  } else {  // line-search trial
+    if (isfinite(ll) && ll > F.ll) {
+      for (int p = 0; p < P; ++p) F.theta[p] = F.trial[p];
+      F.ll = ll;
+      for (int p = 0; p < P; ++p) F.g[p] = g[p];
+      for (int e = 0; e < P * P; ++e) F.M[e] = M[e];
+      for (int c = 0; c < D.q; ++c) F.beta[c] = beta[i * kBQ + c];
+      double sg = 0.0;
+      for (int p = 0; p < P; ++p) sg += F.alpha * F.step[p] * g[p];
+      if (fabs(sg) < F.tol) {
+        F.status = FS_DONE;
+      } else {
+        F.k += 1;
+        fs_begin_iteration(F, P);
+      }
+    } else {
+      F.alpha *= 0.5;
+      F.halv += 1;
+      if (F.halv >= 20) {
+        F.status = FS_DONE;
+      } else {
+        for (int p = 0; p < P; ++p) F.trial[p] = F.theta[p] + F.alpha * F.step[p];
+      }
+    }
+  }
+  status_out[i] = F.status;
 }
 
 template <int KP>
@@ -922,6 +1409,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.nu_code = nu_code;
   a.var = std::exp(theta[0]);
   a.nug = std::exp(theta[1 + d]);
+  a.rec_stride = NC;
   double* tot = ctx->arena.get_t<double>("v.tot", NACC + 1);
   double* dbeta = ctx->arena.get_t<double>("v.beta", 33);
   double* partial2 = ctx->arena.get_t<double>("v.partial2", (size_t)G2 * NV);
@@ -984,4 +1472,353 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   *ll = llv;
   if (jitter_power_max) *jitter_power_max = rounds > 0 ? -13 + rounds : 0;
   (void)precision;
+}
+
+// ---------------------------------------------------------------- batched host side
+
+struct BatchState {
+  bool ready = false;
+  int ndata = 0, dmax = 0, GX = 0, G2X = 0, mmax = 0, nchunk = 0;
+  int64_t nblk = 0, nx = 0, ny = 0, nz = 0;
+  std::vector<DSDesc> desc;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double var_penalty = 1.0;
+};
+
+static BatchState& batch_of(svh_ctx* ctx) {
+  static std::unordered_map<svh_ctx*, BatchState> states;
+  return states[ctx];
+}
+
+void batch_release(svh_ctx* ctx) {
+  BatchState& B = batch_of(ctx);
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B = BatchState();
+}
+
+void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t* q,
+                        const int32_t* nu_code, const int64_t* n, const double* X, const double* y,
+                        const double* Z, const int64_t* nbr, const int32_t* kcnt,
+                        const int32_t* chunk, const int32_t* nchunk, int mmax, const double* lvt,
+                        const int32_t* fixed_nug, const double* log_nug, double lambda_max) {
+  batch_release(ctx);
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(ndata >= 1, "batch: no datasets");
+  SVH_REQUIRE(mmax >= 1 && mmax + 1 <= 32, "batch: conditioning sets must have at most 31 points");
+  B.desc.resize(ndata);
+  int64_t xo = 0, yo = 0, zo = 0, bo = 0, co = 0;
+  int dmax = 1;
+  for (int i = 0; i < ndata; ++i) {
+    SVH_REQUIRE(d[i] >= 1 && d[i] <= 10, "batch: d must be in 1..10");
+    SVH_REQUIRE(q[i] >= 0 && q[i] <= 11, "batch: q must be in 0..11");
+    SVH_REQUIRE(nu_code[i] >= 0 && nu_code[i] <= 2, "batch: nu code must be 0, 1 or 2");
+    SVH_REQUIRE(n[i] >= 1 && n[i] < (1 << 30), "batch: bad n");
+    DSDesc& D = B.desc[i];
+    D.d = d[i];
+    D.q = q[i];
+    D.P = d[i] + 2;
+    D.nu_code = nu_code[i];
+    D.mmax = mmax;
+    D.nchunk = nchunk[i];
+    D.fixed_nug = fixed_nug[i];
+    D.pad = 0;
+    D.xoff = xo;
+    D.yoff = yo;
+    D.zoff = zo;
+    D.blk0 = bo;
+    D.nblk = n[i];
+    D.chunk0 = co;
+    D.lvt = lvt[i];
+    D.log_nug = log_nug[i];
+    D.lam_hi = std::isfinite(lambda_max) ? std::log(lambda_max)
+                                          : std::numeric_limits<double>::infinity();
+    xo += n[i] * d[i];
+    yo += n[i];
+    zo += n[i] * q[i];
+    bo += n[i];
+    co += nchunk[i];
+    dmax = std::max(dmax, d[i]);
+  }
+  B.ndata = ndata;
+  B.dmax = dmax;
+  B.nblk = bo;
+  B.nx = xo;
+  B.ny = yo;
+  B.nz = zo;
+  B.mmax = mmax;
+  B.nchunk = (int)co;
+  int64_t maxn = 0;
+  for (auto& D : B.desc) maxn = std::max(maxn, D.nblk);
+  B.GX = (int)std::min<int64_t>(maxn, 1024);
+  B.G2X = (int)std::min<int64_t>((maxn + 255) / 256, 64);
+  // block tables (rows are 0..n-1 within each dataset, in order)
+  std::vector<int> hrows(bo), hnbr((size_t)bo * mmax, -1), hk(bo), hc(bo), chunk_ds(co);
+  for (int i = 0; i < ndata; ++i) {
+    const DSDesc& D = B.desc[i];
+    for (int64_t r = 0; r < D.nblk; ++r) {
+      const int64_t b = D.blk0 + r;
+      hrows[b] = (int)r;
+      hk[b] = kcnt[b];
+      SVH_REQUIRE(kcnt[b] >= 0 && kcnt[b] <= mmax, "batch: bad neighbour count");
+      SVH_REQUIRE(chunk[b] >= 0 && chunk[b] < D.nchunk, "batch: bad chunk id");
+      hc[b] = (int)(D.chunk0 + chunk[b]);
+      for (int t = 0; t < kcnt[b]; ++t) {
+        const int64_t v = nbr[b * mmax + t];
+        SVH_REQUIRE(v >= 0 && v < D.nblk, "batch: neighbour index out of range");
+        hnbr[(size_t)b * mmax + t] = (int)v;
+      }
+    }
+    for (int c = 0; c < D.nchunk; ++c) chunk_ds[D.chunk0 + c] = i;
+  }
+  hipStream_t s = ctx->stream;
+  auto up = [&](const char* name, const void* src, size_t bytes) {
+    void* dst = ctx->arena.get(name, bytes);
+    if (bytes) SVH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  };
+  up("b.X", X, (size_t)xo * sizeof(double));
+  up("b.y", y, (size_t)yo * sizeof(double));
+  if (zo > 0) up("b.Z", Z, (size_t)zo * sizeof(double));
+  else ctx->arena.get("b.Z", 8);
+  up("b.rows", hrows.data(), hrows.size() * sizeof(int));
+  up("b.nbr", hnbr.data(), hnbr.size() * sizeof(int));
+  up("b.kcnt", hk.data(), hk.size() * sizeof(int));
+  up("b.chunk", hc.data(), hc.size() * sizeof(int));
+  up("b.chunk_ds", chunk_ds.data(), chunk_ds.size() * sizeof(int));
+  up("b.desc", B.desc.data(), B.desc.size() * sizeof(DSDesc));
+  ctx->arena.get("b.level", (size_t)co * sizeof(int));
+  ctx->arena.get("b.fail", (size_t)co * sizeof(int));
+  ctx->arena.get("b.cache", (size_t)bo * kBNC * sizeof(double));
+  ctx->arena.get("b.partial", (size_t)ndata * B.GX * kBNACC * sizeof(double));
+  ctx->arena.get("b.tot", (size_t)ndata * kBNACC * sizeof(double));
+  ctx->arena.get("b.beta", (size_t)ndata * kBQ * sizeof(double));
+  ctx->arena.get("b.partial2", (size_t)ndata * B.G2X * kBNV * sizeof(double));
+  ctx->arena.get("b.out", (size_t)ndata * kBOUT * sizeof(double));
+  ctx->arena.get("b.par", (size_t)ndata * 16 * sizeof(double));
+  ctx->arena.get("b.theval", (size_t)ndata * kBP * sizeof(double));
+  ctx->arena.get("b.fs", (size_t)ndata * sizeof(FSState));
+  ctx->arena.get("b.status", (size_t)ndata * sizeof(int));
+  ctx->stage.get(std::max<size_t>((size_t)ndata * sizeof(int), 64));
+  SVH_HIP(hipStreamSynchronize(s));
+  B.ready = true;
+}
+
+// Replace one dataset's ordered inputs and neighbour lists after a rebuild
+// (same n, so every device buffer keeps its address and the graph stays valid).
+void batch_update_device(svh_ctx* ctx, int i, const double* X, const double* y, const double* Z,
+                         const int64_t* nbr, const int32_t* kcnt) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready && i >= 0 && i < B.ndata, "batch_update: bad dataset");
+  const DSDesc& D = B.desc[i];
+  hipStream_t s = ctx->stream;
+  double* dX = (double*)ctx->arena.get("b.X", 0);
+  double* dy = (double*)ctx->arena.get("b.y", 0);
+  double* dZ = (double*)ctx->arena.get("b.Z", 0);
+  int* dnbr = (int*)ctx->arena.get("b.nbr", 0);
+  int* dk = (int*)ctx->arena.get("b.kcnt", 0);
+  std::vector<int> hnbr((size_t)D.nblk * B.mmax, -1), hk(D.nblk);
+  for (int64_t r = 0; r < D.nblk; ++r) {
+    SVH_REQUIRE(kcnt[r] >= 0 && kcnt[r] <= B.mmax, "batch_update: bad neighbour count");
+    hk[r] = kcnt[r];
+    for (int t = 0; t < kcnt[r]; ++t) {
+      const int64_t v = nbr[r * B.mmax + t];
+      SVH_REQUIRE(v >= 0 && v < D.nblk, "batch_update: neighbour index out of range");
+      hnbr[(size_t)r * B.mmax + t] = (int)v;
+    }
+  }
+  SVH_HIP(hipMemcpyAsync(dX + D.xoff, X, (size_t)D.nblk * D.d * sizeof(double),
+                         hipMemcpyHostToDevice, s));
+  SVH_HIP(hipMemcpyAsync(dy + D.yoff, y, (size_t)D.nblk * sizeof(double), hipMemcpyHostToDevice, s));
+  if (D.q > 0)
+    SVH_HIP(hipMemcpyAsync(dZ + D.zoff, Z, (size_t)D.nblk * D.q * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  SVH_HIP(hipMemcpyAsync(dnbr + D.blk0 * B.mmax, hnbr.data(), hnbr.size() * sizeof(int),
+                         hipMemcpyHostToDevice, s));
+  SVH_HIP(hipMemcpyAsync(dk + D.blk0, hk.data(), hk.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  SVH_HIP(hipStreamSynchronize(s));
+}
+
+// Enqueue one evaluation of every evaluating dataset plus the state update.
+static void batch_enqueue(svh_ctx* ctx, bool with_update) {
+  BatchState& B = batch_of(ctx);
+  hipStream_t s = ctx->stream;
+  const DSDesc* dd = (const DSDesc*)ctx->arena.get("b.desc", 0);
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  double* par = (double*)ctx->arena.get("b.par", 0);
+  double* the = (double*)ctx->arena.get("b.theval", 0);
+  int* level = (int*)ctx->arena.get("b.level", 0);
+  int* fail = (int*)ctx->arena.get("b.fail", 0);
+  const int* chunk_ds = (const int*)ctx->arena.get("b.chunk_ds", 0);
+  double* out = (double*)ctx->arena.get("b.out", 0);
+  double* beta = (double*)ctx->arena.get("b.beta", 0);
+  double* tot = (double*)ctx->arena.get("b.tot", 0);
+  double* partial = (double*)ctx->arena.get("b.partial", 0);
+  double* partial2 = (double*)ctx->arena.get("b.partial2", 0);
+  double* cache = (double*)ctx->arena.get("b.cache", 0);
+  int* status = (int*)ctx->arena.get("b.status", 0);
+  hipLaunchKernelGGL(fs_params, dim3((B.ndata + 63) / 64), dim3(64), 0, s, dd, fs, par, the,
+                     B.ndata);
+  hipLaunchKernelGGL(fs_reset_chunks, dim3((B.nchunk + 255) / 256), dim3(256), 0, s, chunk_ds, fs,
+                     level, fail, B.nchunk);
+  BatchArgs a;
+  a.ds = dd;
+  a.fs = fs;
+  a.par = par;
+  a.X = (const double*)ctx->arena.get("b.X", 0);
+  a.y = (const double*)ctx->arena.get("b.y", 0);
+  a.Z = (const double*)ctx->arena.get("b.Z", 0);
+  a.rows = (const int*)ctx->arena.get("b.rows", 0);
+  a.nbr = (const int*)ctx->arena.get("b.nbr", 0);
+  a.kcnt = (const int*)ctx->arena.get("b.kcnt", 0);
+  a.chunk = (const int*)ctx->arena.get("b.chunk", 0);
+  a.level = level;
+  a.fail = fail;
+  a.cache = cache;
+  a.partial = partial;
+  a.mmax_stride = B.mmax;
+  a.GX = B.GX;
+  const size_t lds = (size_t)split_lds_doubles(B.dmax) * sizeof(double);
+  hipLaunchKernelGGL(ll_pass1_batch, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+  hipLaunchKernelGGL(ll_sum_batch, dim3(kBNACC, B.ndata), dim3(256), 0, s, dd, fs, partial, B.GX,
+                     tot);
+  hipLaunchKernelGGL(ll_beta_batch, dim3(B.ndata), dim3(256), 0, s, dd, fs, tot, beta, out);
+  hipLaunchKernelGGL(ll_pass2_batch, dim3(B.G2X, B.ndata), dim3(256), 0, s, dd, fs, cache, beta,
+                     B.G2X, partial2);
+  hipLaunchKernelGGL(ll_reduce2_batch, dim3(kBNV + 1, B.ndata), dim3(256), 0, s, dd, fs, partial2,
+                     B.G2X, fail, out);
+  if (with_update)
+    hipLaunchKernelGGL(fs_update, dim3(B.ndata), dim3(64), 0, s, dd, fs, out, beta, level, fail,
+                       B.var_penalty, status);
+  SVH_HIP(hipGetLastError());
+}
+
+void batch_fit_start_device(svh_ctx* ctx, const double* theta0, int max_iter, double tol,
+                            double var_penalty, int use_graph) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready, "batch: call svh_batch_setup first");
+  std::vector<FSState> h(B.ndata);
+  for (int i = 0; i < B.ndata; ++i) {
+    FSState& F = h[i];
+    std::memset(&F, 0, sizeof(FSState));
+    for (int p = 0; p < B.desc[i].P; ++p) F.theta[p] = theta0[i * kBP + p];
+    F.status = FS_EVAL_INIT;
+    F.max_iter = max_iter;
+    F.tol = tol;
+    F.alpha = 1.0;
+  }
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  SVH_HIP(hipMemcpyAsync(fs, h.data(), h.size() * sizeof(FSState), hipMemcpyHostToDevice,
+                         ctx->stream));
+  SVH_HIP(hipStreamSynchronize(ctx->stream));
+  B.var_penalty = var_penalty;
+  if (use_graph && !B.exec) {
+    hipStream_t s = ctx->stream;
+    SVH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_enqueue(ctx, true);
+    SVH_HIP(hipStreamEndCapture(s, &B.graph));
+    SVH_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+  }
+}
+
+// Advance every dataset by `nrep` state-machine steps (each = one objective
+// evaluation per evaluating dataset); copies the statuses to `status`.
+void batch_fit_step_device(svh_ctx* ctx, int nrep, int32_t* status) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready, "batch: call svh_batch_setup first");
+  hipStream_t s = ctx->stream;
+  for (int r = 0; r < nrep; ++r) {
+    ctx->timers.begin(s, "batch_step");
+    if (B.exec)
+      SVH_HIP(hipGraphLaunch(B.exec, s));
+    else
+      batch_enqueue(ctx, true);
+    ctx->timers.end(s);
+  }
+  int* dst = (int*)ctx->arena.get("b.status", 0);
+  int* hst = (int*)ctx->stage.get((size_t)B.ndata * sizeof(int));
+  SVH_HIP(hipMemcpyAsync(hst, dst, (size_t)B.ndata * sizeof(int), hipMemcpyDeviceToHost, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  ctx->timers.collect();
+  std::memcpy(status, hst, (size_t)B.ndata * sizeof(int));
+}
+
+void batch_get_device(svh_ctx* ctx, double* theta, double* ll, double* beta, int32_t* status,
+                      int32_t* iters, int32_t* nevals) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready, "batch: call svh_batch_setup first");
+  std::vector<FSState> h(B.ndata);
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  SVH_HIP(hipMemcpyAsync(h.data(), fs, h.size() * sizeof(FSState), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  SVH_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < B.ndata; ++i) {
+    for (int p = 0; p < kBP; ++p) theta[i * kBP + p] = h[i].theta[p];
+    for (int c = 0; c < kBQ; ++c) beta[i * kBQ + c] = h[i].beta[c];
+    ll[i] = h[i].ll;
+    status[i] = h[i].status;
+    iters[i] = h[i].k;
+    nevals[i] = h[i].nevals;
+  }
+}
+
+// Host decision for a dataset in FS_REORDER: 1 = rebuilt (re-evaluate),
+// 0 = unchanged (take the next step without evaluating).
+void batch_set_reorder_device(svh_ctx* ctx, int i, int rebuilt) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready && i >= 0 && i < B.ndata, "batch: bad dataset");
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  const int v = rebuilt ? FS_REEVAL : FS_STEP;
+  SVH_HIP(hipMemcpyAsync((char*)(fs + i) + offsetof(FSState, status), &v, sizeof(int),
+                         hipMemcpyHostToDevice, ctx->stream));
+  SVH_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// One batched objective evaluation at the given parameters (no state
+// update; the jitter ladder runs per dataset as in svh_vecchia_eval).
+void batch_eval_device(svh_ctx* ctx, const double* theta, double* ll, double* grad,
+                       double* fisher, double* beta, int32_t* jitter_rounds) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready, "batch: call svh_batch_setup first");
+  hipStream_t s = ctx->stream;
+  std::vector<FSState> h(B.ndata);
+  for (int i = 0; i < B.ndata; ++i) {
+    std::memset(&h[i], 0, sizeof(FSState));
+    for (int p = 0; p < B.desc[i].P; ++p) h[i].theta[p] = theta[i * kBP + p];
+    h[i].status = FS_EVAL_INIT;
+  }
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  const int* chunk_ds = (const int*)ctx->arena.get("b.chunk_ds", 0);
+  int* level = (int*)ctx->arena.get("b.level", 0);
+  int* fail = (int*)ctx->arena.get("b.fail", 0);
+  double* out = (double*)ctx->arena.get("b.out", 0);
+  double* dbeta = (double*)ctx->arena.get("b.beta", 0);
+  std::vector<double> hout((size_t)B.ndata * kBOUT), hb((size_t)B.ndata * kBQ);
+  int rounds = 0;
+  for (;;) {
+    SVH_HIP(hipMemcpyAsync(fs, h.data(), h.size() * sizeof(FSState), hipMemcpyHostToDevice, s));
+    batch_enqueue(ctx, false);
+    SVH_HIP(hipMemcpyAsync(hout.data(), out, hout.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    SVH_HIP(hipMemcpyAsync(hb.data(), dbeta, hb.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    SVH_HIP(hipStreamSynchronize(s));
+    bool any = false;
+    for (int i = 0; i < B.ndata; ++i) any = any || hout[(size_t)i * kBOUT + kBOUT - 1] > 0.0;
+    if (!any) break;
+    if (rounds == 9) throw SvhError{SVH_ERR_CHOLESKY, "block Cholesky failed even with jitter"};
+    hipLaunchKernelGGL(ll_escalate, dim3((B.nchunk + 255) / 256), dim3(256), 0, s, level, fail,
+                       B.nchunk);
+    for (auto& F : h) F.retry = 1;   // keep the escalated levels on the next pass
+    ++rounds;
+    (void)chunk_ds;
+  }
+  const double LOG2PI = 1.8378770664093453;
+  for (int i = 0; i < B.ndata; ++i) {
+    const DSDesc& D = B.desc[i];
+    const double* o = hout.data() + (size_t)i * kBOUT;
+    ll[i] = -0.5 * (double)D.nblk * LOG2PI + o[0];
+    for (int p = 0; p < D.P; ++p) grad[i * kBP + p] = o[1 + p];
+    for (int e = 0; e < D.P * D.P; ++e) fisher[i * kBP * kBP + e] = o[1 + D.P + e];
+    for (int c = 0; c < D.q; ++c) beta[i * kBQ + c] = hb[(size_t)i * kBQ + c];
+  }
+  if (jitter_rounds) *jitter_rounds = rounds;
 }

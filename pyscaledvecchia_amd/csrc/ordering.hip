@@ -142,11 +142,30 @@ struct MMState {
   int overflow_sorts;
 };
 
+// One ordering problem of a batch (several datasets share each launch).
+struct MMDesc {
+  int64_t xoff;   // dataset's SoA coordinates start at XT + xoff (d x n)
+  int64_t soff;   // s2 / dist / order start at + soff (n entries)
+  int n, d, first, slice0, nslice, pad;
+};
+
 __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
-    const double* __restrict__ XT, int n, int d, double* __restrict__ s2,
-    double* __restrict__ dist, const int* __restrict__ acc, const MMState* __restrict__ st,
-    double* __restrict__ cand_key, int* __restrict__ cand_idx) {
+    const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
+    double* __restrict__ s2all, double* __restrict__ distall, const int* __restrict__ accall,
+    const MMState* __restrict__ stall, double* __restrict__ cand_keyall,
+    int* __restrict__ cand_idxall) {
   extern __shared__ double smem[];
+  const MMDesc D = desc[blockIdx.y];
+  if ((int)blockIdx.x >= D.nslice) return;
+  const int n = D.n, d = D.d;
+  const double* XT = XTall + D.xoff;
+  double* s2 = s2all + D.soff;
+  double* dist = distall + D.soff;
+  const int* acc = accall + (int64_t)blockIdx.y * kAccCap;
+  const MMState* st = stall + blockIdx.y;
+  if (st->count >= n) return;   // order complete: nothing left to select
+  double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
+  int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
   const int A = st->nacc;
   double* accX = smem;  // A * d
   __shared__ double wk[kSlice / kWave];
@@ -222,9 +241,19 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
 }
 
 __global__ __launch_bounds__(kWave) void mm_select_kernel(
-    const double* __restrict__ XT, int n, int d, const double* __restrict__ s2g,
-    int nslice, const double* __restrict__ cand_key, const int* __restrict__ cand_idx,
-    int* __restrict__ acc, MMState* __restrict__ st, int64_t* __restrict__ order, int sortcap) {
+    const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
+    const double* __restrict__ s2all, const double* __restrict__ cand_keyall,
+    const int* __restrict__ cand_idxall, int* __restrict__ accall, MMState* __restrict__ stall,
+    int64_t* __restrict__ orderall, int sortcap) {
+  const MMDesc D = desc[blockIdx.x];
+  const int n = D.n, d = D.d, nslice = D.nslice;
+  const double* XT = XTall + D.xoff;
+  const double* s2g = s2all + D.soff;
+  const double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
+  const int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
+  int* acc = accall + (int64_t)blockIdx.x * kAccCap;
+  MMState* st = stall + blockIdx.x;
+  int64_t* order = orderall + D.soff;
   // LDS: survivors (sortcap keys + sortcap indices)
   extern __shared__ double smem[];
   double* sk = smem;
@@ -406,20 +435,22 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
   }
 }
 
-__global__ void mm_init_kernel(int n, double* s2, double* dist, int* acc, MMState* st,
-                               int64_t* order, int first) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    s2[i] = __longlong_as_double(0x7ff0000000000000ll);  // +inf
-    dist[i] = __longlong_as_double(0x7ff0000000000000ll);
+__global__ void mm_init_kernel(const MMDesc* __restrict__ desc, double* s2all, double* distall,
+                               int* accall, MMState* stall, int64_t* orderall) {
+  const MMDesc D = desc[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D.n) {
+    s2all[D.soff + i] = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+    distall[D.soff + i] = __longlong_as_double(0x7ff0000000000000ll);
   }
   if (i == 0) {
-    acc[0] = first;
-    order[0] = first;
-    st->count = 1;
-    st->nacc = 1;
-    st->rounds = 0;
-    st->overflow_sorts = 0;
+    accall[(int64_t)blockIdx.y * kAccCap] = D.first;
+    orderall[D.soff] = D.first;
+    MMState& st = stall[blockIdx.y];
+    st.count = 1;
+    st.nacc = 1;
+    st.rounds = 0;
+    st.overflow_sorts = 0;
   }
 }
 
@@ -431,14 +462,28 @@ __global__ void mm_init_kernel(int n, double* s2, double* dist, int* acc, MMStat
 // lane); a candidate strictly better than the current k-th best goes to an
 // LDS buffer (ballot + prefix).  When the buffer would overflow, and at the
 // end, the wave bitonic-sorts list+buffer in LDS and keeps the first k.
+// One kNN problem of a batch: ordered (candidates j < row) or query form.
+struct KNNDesc {
+  const double* XT;   // reference coordinates, SoA d x nref
+  const double* QT;   // query coordinates, SoA d x nq (query form)
+  int64_t* out;
+  int64_t nref, nq, rowstart;
+  int d, m, ordered, out_stride;
+};
+
 template <int LCAP>
-__global__ __launch_bounds__(kWave) void knn_kernel(
-    const double* __restrict__ XT, int64_t nref, const double* __restrict__ QT, int64_t nq,
-    int d, int m, int ordered, int64_t rowstart, int64_t* __restrict__ out, int out_stride) {
+__global__ __launch_bounds__(kWave) void knn_kernel(const KNNDesc* __restrict__ descs) {
+  const KNNDesc D = descs[blockIdx.y];
+  const double* __restrict__ XT = D.XT;
+  const double* __restrict__ QT = D.QT;
+  int64_t* __restrict__ out = D.out;
+  const int64_t nref = D.nref, nq = D.nq, rowstart = D.rowstart;
+  const int d = D.d, m = D.m, ordered = D.ordered, out_stride = D.out_stride;
+  const int64_t q = blockIdx.x;
+  if (q >= (ordered ? nref : nq)) return;
   __shared__ double key[LCAP];
   __shared__ int idx[LCAP];
   const int lane = threadIdx.x;
-  const int64_t q = blockIdx.x;
   int64_t hi;
   int k;
   int64_t* orow = out + q * (int64_t)out_stride;
@@ -530,26 +575,33 @@ __global__ __launch_bounds__(kWave) void knn_kernel(
 }
 
 template <int LCAP>
-void launch_knn_t(svh_ctx* ctx, const double* XT, int64_t nref, const double* QT, int64_t nq,
-                  int d, int m, int ordered, int64_t rowstart, int64_t* out, int stride) {
-  int64_t nblocks = ordered ? nref : nq;
-  if (nblocks == 0) return;
+void launch_knn_t(svh_ctx* ctx, const std::vector<KNNDesc>& h, int64_t maxq) {
+  if (maxq == 0 || h.empty()) return;
+  KNNDesc* dd = ctx->arena.get_t<KNNDesc>("knn.desc", h.size());
+  SVH_HIP(hipMemcpyAsync(dd, h.data(), h.size() * sizeof(KNNDesc), hipMemcpyHostToDevice,
+                         ctx->stream));
   ctx->timers.begin(ctx->stream, "knn");
-  hipLaunchKernelGGL((knn_kernel<LCAP>), dim3((unsigned)nblocks), dim3(kWave), 0, ctx->stream, XT,
-                     nref, QT, nq, d, m, ordered, rowstart, out, stride);
+  hipLaunchKernelGGL((knn_kernel<LCAP>), dim3((unsigned)maxq, (unsigned)h.size()), dim3(kWave), 0,
+                     ctx->stream, dd);
   SVH_HIP(hipGetLastError());
   ctx->timers.end(ctx->stream);
 }
 
-void launch_knn_any(svh_ctx* ctx, const double* XT, int64_t nref, const double* QT, int64_t nq,
-                    int d, int m, int ordered, int64_t rowstart, int64_t* out, int stride) {
-  SVH_REQUIRE(m <= 448, "k/m above 448 not supported by the kNN kernel");
-  if (m <= 64)
-    launch_knn_t<128>(ctx, XT, nref, QT, nq, d, m, ordered, rowstart, out, stride);
-  else if (m <= 192)
-    launch_knn_t<256>(ctx, XT, nref, QT, nq, d, m, ordered, rowstart, out, stride);
+void launch_knn_descs(svh_ctx* ctx, const std::vector<KNNDesc>& h) {
+  int mmax = 0;
+  int64_t maxq = 0;
+  for (const auto& D : h) {
+    SVH_REQUIRE(D.m <= 448, "k/m above 448 not supported by the kNN kernel");
+    mmax = std::max(mmax, D.m);
+    maxq = std::max<int64_t>(maxq, D.ordered ? D.nref : D.nq);
+  }
+  SVH_REQUIRE(maxq < (1ll << 31), "kNN: too many queries");
+  if (mmax <= 64)
+    launch_knn_t<128>(ctx, h, maxq);
+  else if (mmax <= 192)
+    launch_knn_t<256>(ctx, h, maxq);
   else
-    launch_knn_t<512>(ctx, XT, nref, QT, nq, d, m, ordered, rowstart, out, stride);
+    launch_knn_t<512>(ctx, h, maxq);
 }
 
 }  // namespace
@@ -565,32 +617,41 @@ void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const doubl
   SVH_HIP(hipGetLastError());
 }
 
-void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t first,
-                    int64_t* dorder) {
-  SVH_REQUIRE(n64 < INT_MAX / 2, "maximin: n too large");
-  const int n = (int)n64;
-  if (n == 0) return;
-  const int nslice = (n + kSlice - 1) / kSlice;
-  double* s2 = ctx->arena.get_t<double>("mm.s2", n);
-  double* dist = ctx->arena.get_t<double>("mm.dist", n);
-  int* acc = ctx->arena.get_t<int>("mm.acc", kAccCap);
-  MMState* st = ctx->arena.get_t<MMState>("mm.st", 1);
-  double* ck = ctx->arena.get_t<double>("mm.ck", (size_t)nslice * (kSliceTop + 1));
-  int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice * (kSliceTop + 1));
-  hipLaunchKernelGGL(mm_init_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, s2,
-                     dist, acc, st, dorder, (int)first);
+// Maximin orders of several problems at once: rounds of (apply, select)
+// launches over all unfinished problems until every order is complete.
+void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMDesc>& hd,
+                          int64_t* dorder) {
+  const int nd = (int)hd.size();
+  if (nd == 0) return;
+  int64_t tot = 0;
+  int maxslice = 0, dmax = 1, maxn = 0, nslice_tot = 0;
+  for (const auto& D : hd) {
+    tot = std::max<int64_t>(tot, D.soff + D.n);
+    maxslice = std::max(maxslice, D.nslice);
+    dmax = std::max(dmax, D.d);
+    maxn = std::max(maxn, D.n);
+    nslice_tot = std::max(nslice_tot, D.slice0 + D.nslice);
+  }
+  double* s2 = ctx->arena.get_t<double>("mm.s2", tot);
+  double* dist = ctx->arena.get_t<double>("mm.dist", tot);
+  int* acc = ctx->arena.get_t<int>("mm.acc", (size_t)nd * kAccCap);
+  MMState* st = ctx->arena.get_t<MMState>("mm.st", nd);
+  double* ck = ctx->arena.get_t<double>("mm.ck", (size_t)nslice_tot * (kSliceTop + 1));
+  int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTop + 1));
+  MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
+  SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
+                     s2, dist, acc, st, dorder);
   SVH_HIP(hipGetLastError());
-  if (n == 1) return;
-  const size_t apply_lds = (size_t)kAccCap * d * sizeof(double);
+  if (maxn <= 1) return;
+  const size_t apply_lds = (size_t)kAccCap * dmax * sizeof(double);
+  SVH_REQUIRE(apply_lds <= 64 * 1024 && dmax <= kMMD, "maximin: d above 12 not supported");
   int sortcap = 1;
-  while (sortcap < nslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
+  while (sortcap < maxslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
   const size_t select_lds =
       (size_t)sortcap * sizeof(double) + (size_t)((sortcap + 1) / 2) * sizeof(double);
-  SVH_REQUIRE(select_lds <= 150 * 1024, "maximin: d too large for the selection pool");
-  SVH_REQUIRE(apply_lds <= 64 * 1024 && d <= kMMD, "maximin: d above 12 not supported");
-  MMState* hst = static_cast<MMState*>(ctx->stage.get(sizeof(MMState)));
-  // enqueue rounds in batches; each round accepts at least one point
+  MMState* hst = static_cast<MMState*>(ctx->stage.get(nd * sizeof(MMState)));
   int batch = 8;
   hipEvent_t mm_t0 = nullptr, mm_t1 = nullptr;
   if (ctx->timers.enabled) {
@@ -601,18 +662,20 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
   for (;;) {
     for (int r = 0; r < batch; ++r) {
       ctx->timers.begin(ctx->stream, "maximin_apply");
-      hipLaunchKernelGGL(mm_apply_kernel, dim3(nslice), dim3(kSlice), apply_lds, ctx->stream,
-                         dXT, n, d, s2, dist, acc, st, ck, ci);
+      hipLaunchKernelGGL(mm_apply_kernel, dim3(maxslice, nd), dim3(kSlice), apply_lds, ctx->stream,
+                         dd, dXT, s2, dist, acc, st, ck, ci);
       ctx->timers.end(ctx->stream);
       ctx->timers.begin(ctx->stream, "maximin_select");
-      hipLaunchKernelGGL(mm_select_kernel, dim3(1), dim3(kWave), select_lds, ctx->stream, dXT,
-                         n, d, s2, nslice, ck, ci, acc, st, dorder, sortcap);
+      hipLaunchKernelGGL(mm_select_kernel, dim3(nd), dim3(kWave), select_lds, ctx->stream, dd, dXT,
+                         s2, ck, ci, acc, st, dorder, sortcap);
       ctx->timers.end(ctx->stream);
     }
     SVH_HIP(hipGetLastError());
-    SVH_HIP(hipMemcpyAsync(hst, st, sizeof(MMState), hipMemcpyDeviceToHost, ctx->stream));
+    SVH_HIP(hipMemcpyAsync(hst, st, nd * sizeof(MMState), hipMemcpyDeviceToHost, ctx->stream));
     SVH_HIP(hipStreamSynchronize(ctx->stream));
-    if (hst->count >= n) break;
+    bool done = true;
+    for (int i = 0; i < nd; ++i) done = done && hst[i].count >= hd[i].n;
+    if (done) break;
     if (batch < 64) batch *= 2;
   }
   if (mm_t0) {
@@ -628,15 +691,93 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
   }
   SVH_HIP(hipStreamSynchronize(ctx->stream));
   ctx->timers.collect();
-  ctx->mm_rounds = hst->rounds;
+  int rounds = 0;
+  for (int i = 0; i < nd; ++i) rounds = std::max(rounds, hst[i].rounds);
+  ctx->mm_rounds = rounds;
+}
+
+void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t first,
+                    int64_t* dorder) {
+  SVH_REQUIRE(n64 < INT_MAX / 2, "maximin: n too large");
+  MMDesc D;
+  D.xoff = 0;
+  D.soff = 0;
+  D.n = (int)n64;
+  D.d = d;
+  D.first = (int)first;
+  D.slice0 = 0;
+  D.nslice = (D.n + kSlice - 1) / kSlice;
+  D.pad = 0;
+  launch_maximin_batch(ctx, dXT, std::vector<MMDesc>{D}, dorder);
 }
 
 void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m, int64_t start,
                         int64_t* dnn) {
-  launch_knn_any(ctx, dXT, n, nullptr, 0, d, m, 1, start < 1 ? 1 : start, dnn, m + 1);
+  KNNDesc D{dXT, nullptr, dnn, n, 0, start < 1 ? 1 : start, d, m, 1, m + 1};
+  launch_knn_descs(ctx, std::vector<KNNDesc>{D});
 }
 
 void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const double* dXqT,
                       int64_t nq, int d, int k, int64_t* dnb, int nb_stride) {
-  launch_knn_any(ctx, dXrefT, nref, dXqT, nq, d, k, 0, 0, dnb, nb_stride);
+  KNNDesc D{dXrefT, dXqT, dnb, nref, nq, 0, d, k, 0, nb_stride};
+  launch_knn_descs(ctx, std::vector<KNNDesc>{D});
+}
+
+// Several datasets at once: maximin orders (first points given) and ordered
+// kNN of the reordered inputs.  X holds the datasets' rows back to back.
+void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t* d,
+                           const int64_t* first, const double* X, int m, int64_t* order,
+                           int64_t* nn) {
+  std::vector<MMDesc> hm(nd);
+  int64_t xo = 0, so = 0;
+  int sl = 0;
+  for (int i = 0; i < nd; ++i) {
+    SVH_REQUIRE(n[i] >= 1 && n[i] < (1 << 29) && d[i] >= 1 && d[i] <= kMMD,
+                "ordering_batch: bad n or d");
+    SVH_REQUIRE(first[i] >= 0 && first[i] < n[i], "ordering_batch: first out of range");
+    MMDesc& D = hm[i];
+    D.xoff = xo;
+    D.soff = so;
+    D.n = (int)n[i];
+    D.d = d[i];
+    D.first = (int)first[i];
+    D.slice0 = sl;
+    D.nslice = (D.n + kSlice - 1) / kSlice;
+    D.pad = 0;
+    xo += n[i] * d[i];
+    so += n[i];
+    sl += D.nslice;
+  }
+  hipStream_t s = ctx->stream;
+  double* dX = ctx->arena.get_t<double>("ob.X", xo);
+  double* dXT = ctx->arena.get_t<double>("ob.XT", xo);
+  double* dXoT = ctx->arena.get_t<double>("ob.XoT", xo);
+  int64_t* dord = ctx->arena.get_t<int64_t>("ob.order", so);
+  int64_t* dnn = ctx->arena.get_t<int64_t>("ob.nn", (size_t)so * (m + 1));
+  SVH_HIP(hipMemcpyAsync(dX, X, (size_t)xo * sizeof(double), hipMemcpyHostToDevice, s));
+  for (int i = 0; i < nd; ++i)
+    launch_to_soa(ctx, dX + hm[i].xoff, n[i], d[i], nullptr, dXT + hm[i].xoff);
+  launch_maximin_batch(ctx, dXT, hm, dord);
+  SVH_HIP(hipMemcpyAsync(order, dord, (size_t)so * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  // reordered inputs (host permutation keeps the exact values), then kNN
+  std::vector<double> Xo((size_t)xo);
+  for (int i = 0; i < nd; ++i) {
+    const int64_t base = hm[i].xoff, ob = hm[i].soff;
+    for (int64_t r = 0; r < n[i]; ++r) {
+      const int64_t src = order[ob + r];
+      for (int l = 0; l < d[i]; ++l) Xo[base + r * d[i] + l] = X[base + src * d[i] + l];
+    }
+  }
+  SVH_HIP(hipMemcpyAsync(dX, Xo.data(), (size_t)xo * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<KNNDesc> hk(nd);
+  for (int i = 0; i < nd; ++i) {
+    launch_to_soa(ctx, dX + hm[i].xoff, n[i], d[i], nullptr, dXoT + hm[i].xoff);
+    hk[i] = KNNDesc{dXoT + hm[i].xoff, nullptr, dnn + hm[i].soff * (m + 1), n[i], 0, 1, d[i], m, 1,
+                    m + 1};
+  }
+  launch_knn_descs(ctx, hk);
+  SVH_HIP(hipMemcpyAsync(nn, dnn, (size_t)so * (m + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  ctx->timers.collect();
 }

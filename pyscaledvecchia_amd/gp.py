@@ -118,7 +118,10 @@ class ScaledVecchiaGP:
                 "general or estimated nu is not supported")
 
     # ---------------- fitting (gp.py:138-239) -----------------------------
-    def fit(self, X, y):
+    def _prepare(self, X, y):
+        """Validation, standardisation, subsample and initial parameters
+        (gp.py:139-177).  Returns the estimation data and the numpy RNG in the
+        state the reference leaves it (later used by the b-step)."""
         X = np.ascontiguousarray(np.atleast_2d(X), dtype=float)
         y = np.asarray(y, dtype=float).ravel()
         n, d = X.shape
@@ -157,6 +160,38 @@ class ScaledVecchiaGP:
             theta[1:1 + d] = math.log(0.2 * math.sqrt(d))
             theta[-1] = math.log(self.nugget if fixed_nug else 0.1)
         theta = self._clip_theta(theta, d)
+        return dict(n=n, d=d, rng=rng, Xe=Xe, ye=ye, Ze=Ze, m=m, theta=theta,
+                    fixed_nug=fixed_nug, lvt=math.log(max(ye.var(), 1e-8)))
+
+    def _order_structure(self, Xe, theta, m):
+        """Maximin order and ordered neighbours in the scaled space
+        (gp.py:182-191), on the GPU."""
+        d = Xe.shape[1]
+        Xs = Xe / np.exp(theta[1:1 + d])
+        order = maximin_order(Xs)
+        nn = find_ordered_nn(Xs[order], m)
+        return order, nn
+
+    def _finish(self, theta, ll, beta, rng, n):
+        """Final clip, fitted attributes and the b-step (gp.py:227-239)."""
+        d = self.d_
+        theta = self._clip_theta(np.array(theta), d)
+        if self.nugget is not None:
+            theta[-1] = math.log(self.nugget)
+        self.theta_ = theta
+        self.beta_ = beta
+        self.loglik_ = ll
+        self._nu_fit = self.nu
+        self.b_ = 1.0
+        if self.var_correction and n >= 200:
+            self.b_ = self._estimate_b(rng)
+        return self
+
+    def fit(self, X, y):
+        prep = self._prepare(X, y)
+        n, d, rng = prep["n"], prep["d"], prep["rng"]
+        Xe, ye, Ze, m, theta = prep["Xe"], prep["ye"], prep["Ze"], prep["m"], prep["theta"]
+        fixed_nug = prep["fixed_nug"]
 
         ctx = self._ctx()
         state = {}
@@ -164,9 +199,7 @@ class ScaledVecchiaGP:
         self.n_evals_ = 0
 
         def rebuild(th):
-            Xs = Xe / np.exp(th[1:1 + d])
-            order = maximin_order(Xs)
-            nn = find_ordered_nn(Xs[order], m)
+            order, nn = self._order_structure(Xe, th, m)
             groups = _nn_groups(nn, m)
             state["order"] = order
             state["groups"] = groups
@@ -175,7 +208,7 @@ class ScaledVecchiaGP:
 
         rebuild(theta)
         last_ranges = np.exp(theta[1:1 + d]).copy()
-        log_var_target = math.log(max(ye.var(), 1e-8))
+        log_var_target = prep["lvt"]
 
         def objective(th):
             th = self._clip_theta(np.array(th, dtype=float), d)
@@ -206,18 +239,8 @@ class ScaledVecchiaGP:
         theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
                                           tol=self.tol, verbose=self.verbose,
                                           reorder_hook=reorder_hook, history=self.history_)
-        theta = self._clip_theta(np.array(theta), d)
-        if fixed_nug:
-            theta[-1] = math.log(self.nugget)
-        self.theta_ = theta
-        self.beta_ = beta
-        self.loglik_ = ll
-        self._nu_fit = self.nu
         self._fit_state = state
-        self.b_ = 1.0
-        if self.var_correction and n >= 200:
-            self.b_ = self._estimate_b(rng)
-        return self
+        return self._finish(theta, ll, beta, rng, n)
 
     # ---------------- prediction (gp.py:242-304) ----------------------------
     def _predict_scaled(self, Xs_std, X_tr, y_tr_resid, m, noise_free=True):

@@ -1,0 +1,106 @@
+"""Batched fits (many uneven datasets in one launch) against one-at-a-time fits
+and the oracle.  GPU only."""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import vecchia_np as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _datasets(seed, nd=6):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nd):
+        n = int(rng.integers(300, 1500))
+        d = int(rng.integers(2, 7))
+        X = rng.random((n, d))
+        w = rng.standard_normal(d)
+        y = np.sin(3 * X @ w) + 0.3 * X[:, 0] ** 2 + 0.05 * rng.standard_normal(n)
+        out.append((X, y))
+    return out
+
+
+def test_batch_eval_matches_single_structure(gpu):
+    from pyscaledvecchia_amd import _lib
+    from pyscaledvecchia_amd.batch import BatchedFit
+    from pyscaledvecchia_amd.gp import ScaledVecchiaGP
+    data = _datasets(1, nd=5)
+    nus = [0.5, 1.5, 2.5, 2.5, 1.5]
+    trends = ["constant", "linear", "zero", "linear", "constant"]
+    models = [ScaledVecchiaGP(m_est=20, nu=nu, trend=t, n_est=2000) for nu, t in zip(nus, trends)]
+    bf = BatchedFit(models)
+    preps = [g._prepare(X, y) for g, (X, y) in zip(models, data)]
+    structs = [bf._structure(g, p, p["theta"], 20) for g, p in zip(models, preps)]
+    bh = _lib.BatchHandle(gpu)
+    q = [s["Zo"].shape[1] for s in structs]
+    bh.setup([p["d"] for p in preps], q, [{0.5: 0, 1.5: 1, 2.5: 2}[nu] for nu in nus],
+             [len(s["yo"]) for s in structs], np.concatenate([s["Xo"].ravel() for s in structs]),
+             np.concatenate([s["yo"] for s in structs]),
+             np.concatenate([s["Zo"].ravel() for s in structs]),
+             np.concatenate([s["nbr"] for s in structs]),
+             np.concatenate([s["kcnt"] for s in structs]),
+             np.concatenate([s["chunk"] for s in structs]), [s["nchunk"] for s in structs], 20,
+             [p["lvt"] for p in preps], [0] * 5, [0.0] * 5, 1e3)
+    rng = np.random.default_rng(3)
+    th = np.zeros((5, 12))
+    for i, p in enumerate(preps):
+        th[i, :p["d"] + 2] = p["theta"] + 0.3 * rng.standard_normal(p["d"] + 2)
+    ll, g, F, beta, jr = bh.eval(th)
+    for i, (s, p) in enumerate(zip(structs, preps)):
+        P = p["d"] + 2
+        groups = O.nn_groups(np.column_stack([np.arange(len(s["yo"])), s["nbr"][:, :20]]), 20)
+        Zarg = s["Zo"] if q[i] else None
+        ll0, g0, F0, b0 = O.vecchia_profile_loglik(th[i, :P], s["Xo"], s["yo"], Zarg, groups, nus[i])
+        assert abs(ll[i] - ll0) <= 1e-10 * abs(ll0)
+        np.testing.assert_allclose(g[i, :P], g0, rtol=1e-8, atol=1e-9 * np.abs(g0).max())
+        np.testing.assert_allclose(F[i, :P * P].reshape(P, P), F0, rtol=1e-8,
+                                   atol=1e-9 * np.abs(F0).max())
+        if q[i]:
+            np.testing.assert_allclose(beta[i, :q[i]], b0, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batched_fit_equals_individual_fits(gpu, use_graph):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    from pyscaledvecchia_amd.batch import BatchedFit
+    data = _datasets(7, nd=6)
+    nus = [2.5, 1.5, 0.5, 2.5, 1.5, 2.5]
+    kw = dict(m_est=15, m_pred=30, n_est=1000, trend="constant", var_correction=True)
+    singles = [ScaledVecchiaGP(nu=nu, **kw).fit(X, y) for nu, (X, y) in zip(nus, data)]
+    batch = [ScaledVecchiaGP(nu=nu, **kw) for nu in nus]
+    bf = BatchedFit(batch, use_graph=use_graph)
+    bf.fit([X for X, _ in data], [y for _, y in data])
+    for s, b in zip(singles, batch):
+        assert b.loglik_ == pytest.approx(s.loglik_, rel=1e-9)
+        np.testing.assert_allclose(b.theta_, s.theta_, rtol=1e-7, atol=1e-7)
+        np.testing.assert_allclose(b.beta_, s.beta_, rtol=1e-7, atol=1e-9)
+        assert b.n_evals_ == s.n_evals_
+        assert b.b_ == pytest.approx(s.b_, rel=1e-7)
+    assert bf.stats["steps"] > 0
+
+
+def test_batched_fit_fixed_nugget_and_linear_trend(gpu):
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    from pyscaledvecchia_amd.batch import fit_batch
+    data = _datasets(11, nd=3)
+    kw = dict(m_est=10, n_est=800, trend="linear", nugget=1e-6)
+    models, stats = fit_batch([X for X, _ in data], [y for _, y in data], **kw)
+    for m, (X, y) in zip(models, data):
+        s = ScaledVecchiaGP(**kw).fit(X, y)
+        assert m.nugget_ == pytest.approx(1e-6, rel=1e-9)
+        assert m.loglik_ == pytest.approx(s.loglik_, rel=1e-9)
+        np.testing.assert_allclose(m.predict(X[:10]), s.predict(X[:10]), rtol=1e-7, atol=1e-9)
+
+
+def test_ordering_batch_bit_exact(gpu):
+    rng = np.random.default_rng(5)
+    Xs = [rng.random((n, d)) / 0.4 for n, d in [(700, 3), (1500, 8), (50, 2), (2100, 5)]]
+    res = gpu.ordering_batch(Xs, 12)
+    for X, (order, nn) in zip(Xs, res):
+        ref = O.maximin_order(X)
+        np.testing.assert_array_equal(order, ref)
+        np.testing.assert_array_equal(nn, O.find_ordered_nn(X[ref], 12))
