@@ -77,6 +77,7 @@ struct LLArgs {
   int d, q, mmax, P, need_grad, nu_code;
   double var, nug;
   int rec_stride;  // doubles between consecutive block records
+  unsigned long long* prof;  // per-phase shader-clock sums (profiling runs), or NULL
 };
 
 __device__ __forceinline__ void matern(int nu_code, double r, double& M, double& g) {
@@ -371,8 +372,17 @@ __host__ __device__ inline int split_lds_doubles(int d) {
 // the batched kernels).  Accumulates this block's Fisher / GLS contributions
 // into the lane's task accumulators and writes the block record.
 template <int NU>
+#define SVH_PHASE(k)                                   \
+  if (a.prof) {                                        \
+    const unsigned long long t_ = __builtin_amdgcn_s_memtime(); \
+    ph[k] += t_ - ph_t;                                \
+    ph_t = t_;                                         \
+  }
+
 __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* lds,
-                                            double (&accF)[2], double (&accX)[3]) {
+                                            double (&accF)[2], double (&accX)[3],
+                                            unsigned long long (&ph)[8]) {
+  unsigned long long ph_t = a.prof ? __builtin_amdgcn_s_memtime() : 0ull;
   const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
@@ -399,6 +409,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   double xr[kSD];
 #pragma unroll
   for (int l = 0; l < kSD; ++l) xr[l] = (l < d) ? Xs[r * d + l] : 0.0;
+  SVH_PHASE(0)
   // ---- pairs: Sigma lower, g upper
 #pragma unroll 1
   for (int t = 9 * h; t < 9 * h + 9 && t <= 16; ++t) {
@@ -428,6 +439,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
     const int c = kSH * h + cc;
     S[cc] = (c <= r) ? Mt[r * 33 + c] : Mt[c * 33 + r];
   }
+  SVH_PHASE(1)
   // ---- chunk jitter (rare): Sigma += 10^p tr(Sigma)/K I on the real rows
   double jit = 0.0;
   if (lev > 0) {
@@ -473,6 +485,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
     Misc[2] = 0.5 * log(piv);
   }
   __syncthreads();
+  SVH_PHASE(2)
   // ---- a = L^-T e_K:  L'^T at = e_K, a = at / L_KK  (both halves, same values)
   double at = (r == K - 1) ? 1.0 : 0.0;
 #pragma unroll 4
@@ -485,31 +498,35 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   const double av = at * dK;
   if (h == 0) Cb[r] = av;
   __syncthreads();
+  SVH_PHASE(3)
   // ---- right-hand sides
   double R[kSC];
 #pragma unroll
   for (int c = 0; c < kSC; ++c) R[c] = 0.0;
   if (grad) {
-    // dS_{1+l} a for this half's dims, then half 1 gathers all of them
+    // dS_{1+l} a: half h sums j in [16h, 16h+16) for every dim, the halves
+    // are added, and half 1 keeps the result
     double accl[kSD];
 #pragma unroll
     for (int l = 0; l < kSD; ++l) accl[l] = 0.0;
-#pragma unroll 2
-    for (int j = 0; j < kSK; ++j) {
-      const double gj = (j == r) ? 0.0 : ((j < r) ? Mt[j * 33 + r] : Mt[r * 33 + j]);
+#pragma unroll 4
+    for (int jj = 0; jj < kSH; ++jj) {
+      const int j = kSH * h + jj;
+      const double* gp = (j < r) ? &Mt[j * 33 + r] : &Mt[r * 33 + j];
+      const double gj = (j == r) ? 0.0 : *gp;
       const double wj = gj * Cb[j];
+      const double* xj = Xs + j * d;
 #pragma unroll
       for (int l = 0; l < kSD; ++l)
-        if (l >= dlo && l < dhi) {
-          const double dd = xr[l] - Xs[j * d + l];
+        if (l < d) {
+          const double dd = xr[l] - xj[l];
           accl[l] = fma(wj * dd, dd, accl[l]);
         }
     }
 #pragma unroll
     for (int l = 0; l < kSD; ++l) {
-      const double other = __shfl_xor(accl[l], 32, 64);
-      const double full = (l >= dlo && l < dhi) ? accl[l] : other;
-      if (l < d) R[1 + l] = h ? -a.var * full : R[1 + l];
+      const double full = accl[l] + __shfl_xor(accl[l], 32, 64);
+      if (l < d && h == 1) R[1 + l] = -a.var * full;
     }
     if (h == 1) {
       R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
@@ -522,6 +539,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
     for (int c = 0; c < kSQ; ++c)
       if (c < q) R[1 + c] = a.Z[(int64_t)pt * q + c];
   }
+  SVH_PHASE(4)
   // ---- forward solve with the unit factor L', then scale by 1/L_rr
   const int ncol = grad ? 2 : 1;   // halves that carry columns
 #pragma unroll 1
@@ -543,6 +561,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   const bool ok = !fail;
   if (fail && lane == 0) atomicOr(a.fail + ch, 1);
   __syncthreads();
+  SVH_PHASE(5)
   // ---- columns to LDS by parameter index: W_p (p < P), then yt, Zt_c
   const int cyl = grad ? P : 0;
   if (h == 1 && grad) {
@@ -619,6 +638,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
         accX[2] += v;
     }
   }
+  SVH_PHASE(6)
 }
 
 template <int NU>
@@ -631,7 +651,11 @@ __global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
   const int nX = q * q + q;
   double accF[2] = {0.0, 0.0};   // Fisher tasks lane, lane+64 (nF <= 78)
   double accX[3] = {0.0, 0.0, 0.0};  // GLS tasks lane, +64, +128 (nX <= 132)
-  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x) split_block<NU>(a, b, lds, accF, accX);
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x)
+    split_block<NU>(a, b, lds, accF, accX, ph);
+  if (a.prof && lane == 0)
+    for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   // this workgroup's partials (layout of ll_pass1: F (P*P), XtX (q*q), Zty (q))
   const int NACC = grad ? P * P + q * q + q : q * q + q;
   const int X0 = grad ? P * P : 0;
@@ -932,7 +956,10 @@ __global__ __launch_bounds__(kWave) void ll_pass1_batch(BatchArgs B) {
   a.rec_stride = kBNC;
   double accF[2] = {0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
-  for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX) split_block<-1>(a, b, lds, accF, accX);
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  a.prof = nullptr;
+  for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
+    split_block<-1>(a, b, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
   const int nF = P * (P + 1) / 2, nX = q * q + q;
   double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
@@ -1410,6 +1437,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.var = std::exp(theta[0]);
   a.nug = std::exp(theta[1 + d]);
   a.rec_stride = NC;
+  a.prof = nullptr;
+  const char* penv = std::getenv("SVH_PHASE_PROFILE");
+  if (penv && use_split) {
+    a.prof = ctx->arena.get_t<unsigned long long>("v.prof", 8);
+    hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, (int*)a.prof, 0, (int64_t)16);
+  }
   double* tot = ctx->arena.get_t<double>("v.tot", NACC + 1);
   double* dbeta = ctx->arena.get_t<double>("v.beta", 33);
   double* partial2 = ctx->arena.get_t<double>("v.partial2", (size_t)G2 * NV);
@@ -1471,6 +1504,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   }
   *ll = llv;
   if (jitter_power_max) *jitter_power_max = rounds > 0 ? -13 + rounds : 0;
+  if (a.prof) {
+    unsigned long long hp[8];
+    SVH_HIP(hipMemcpy(hp, a.prof, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    std::fprintf(stderr, "[svh phase clocks] gather %llu pairs %llu chol %llu back %llu rhs %llu "
+                 "solve %llu dots %llu\n", hp[0], hp[1], hp[2], hp[3], hp[4], hp[5], hp[6]);
+  }
   (void)precision;
 }
 
