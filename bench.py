@@ -120,6 +120,34 @@ def block_flops(K, d, P, q):
             + 2 * K * (1 + q) + q * (q + 1) + 2 * q + K ** 2 + 2 * K * q + 2 * P * K)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ll_pass1_split_c2_traffic.json")
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """A plain device-to-device copy of known size: the HBM counters
+    (FETCH_SIZE, WRITE_SIZE) read for it give the scale between counter
+    units and bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "check": float(b[-1].item())}))
+
+
+def measured_traffic():
+    """HBM bytes per launch of the likelihood kernel from the committed
+    counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration),
+    or None when no such summary exists."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +161,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
     ap.add_argument("--ndata", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.workload == "c3":
         run_c3(args)
         return
@@ -211,9 +244,9 @@ def main():
         "config": {"workload": f"C2 objective eval n={n} d={d} m_est={m} nu=2.5 trend=linear",
                    "global_batch": n, "seq_len": m + 1, "parallelism": "single-gpu",
                    "precision": args.precision},
-        "roofline": {"bound": "mfma", "kernel": "ll_pass1", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "ll_pass1_split", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
                      "kernel_avg_ms": 1e3 * k_avg_s, "flops_per_launch": flops},
         "loglik": ll,
         "rebuild_s": {"maximin": t_mm, "ordered_knn": t_knn},
