@@ -210,18 +210,27 @@ def main():
     st = VecchiaStructure(Xe[order], ye[order], Ze[order], groups, ctx=ctx)
     P, q = d + 2, d + 1
 
-    for _ in range(args.warmup):
-        st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=args.precision)
-    ctx.timing(True)
-    ctx.timing_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ll, g, F, beta = st(theta, 2.5, var_penalty=1.0, log_var_target=lvt,
-                            precision=args.precision)
-    t_steps = time.perf_counter() - t0     # every call synchronises on return
-    ctx.timing(False)
-    k_ms, k_cnt = ctx.timing_read("vecchia_pass1")
+    def timed_evals(precision):
+        for _ in range(args.warmup):
+            st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
+        t = time.perf_counter() - t0     # every call synchronises on return
+        ctx.timing(False)
+        return res, t, ctx.timing_read("vecchia_pass1")
+
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    (ll, g, F, beta), t_steps, (k_ms, k_cnt) = timed_evals(args.precision)
+    (ll_o, g_o, F_o, _), t_o, (k_ms_o, k_cnt_o) = timed_evals(other)
     ms_step = 1e3 * t_steps / args.steps
+    g64, F64, ll64 = (g, F, ll) if args.precision == "fp64" else (g_o, F_o, ll_o)
+    gmx, Fmx, llmx = (g_o, F_o, ll_o) if args.precision == "fp64" else (g, F, ll)
+
+    def rel(a, b):
+        return float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
     ks = np.minimum(np.arange(n), m) + 1
     flops = float(sum(block_flops(int(K), d, P, q) for K in ks[:m + 1]) +
                   (n - m - 1) * block_flops(m + 1, d, P, q))
@@ -249,25 +258,41 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
                      "kernel_avg_ms": 1e3 * k_avg_s, "flops_per_launch": flops},
         "loglik": ll,
+        other: {"evals_per_s": args.steps / t_o, "ms_per_step": 1e3 * t_o / args.steps,
+                "kernel_avg_ms": k_ms_o / max(k_cnt_o, 1)},
+        "mixed_vs_fp64_rel_err": {"loglik": abs(llmx - ll64) / abs(ll64),
+                                  "grad": rel(gmx, g64), "fisher": rel(Fmx, F64)},
         "rebuild_s": {"maximin": t_mm, "ordered_knn": t_knn},
         "maximin_detail": mm_detail,
     }
     # full fit of the same problem (host control loop + GPU hot path)
     if not args.no_fit:
-        ctx.timing(True)
-        ctx.timing_reset()
-        t0 = time.perf_counter()
-        gp = ScaledVecchiaGP(m_est=m, m_pred=140, nu=2.5, trend="linear", n_est=n,
-                             precision=args.precision).fit(X, y)
-        t_fit = time.perf_counter() - t0
-        ctx.timing(False)
-        out["fit_s"] = t_fit
-        out["fit"] = {"evals": gp.n_evals_, "rebuilds": gp.n_rebuilds_,
-                      "loglik": gp.loglik_, "b": gp.b_,
-                      "maximin_s": ctx.timing_read("maximin")[0] / 1e3,
-                      "knn_s": ctx.timing_read("knn")[0] / 1e3,
-                      "pass1_s": ctx.timing_read("vecchia_pass1")[0] / 1e3,
-                      "predict_s": ctx.timing_read("predict_blocks")[0] / 1e3}
+        Xt = np.random.default_rng(1012).random((2000, d))
+        fits = {}
+        for prec in (args.precision, other):
+            ctx.timing(True)
+            ctx.timing_reset()
+            t0 = time.perf_counter()
+            gp = ScaledVecchiaGP(m_est=m, m_pred=140, nu=2.5, trend="linear", n_est=n,
+                                 precision=prec).fit(X, y)
+            t_fit = time.perf_counter() - t0
+            ctx.timing(False)
+            mu, sd = gp.predict(Xt, return_std=True)
+            fits[prec] = (gp, mu, sd)
+            out["fit_s" if prec == args.precision else f"fit_s_{prec}"] = t_fit
+            out["fit" if prec == args.precision else f"fit_{prec}"] = {
+                "evals": gp.n_evals_, "rebuilds": gp.n_rebuilds_,
+                "refine_iters": getattr(gp, "n_refine_", 0),
+                "loglik": gp.loglik_, "b": gp.b_,
+                "maximin_s": ctx.timing_read("maximin")[0] / 1e3,
+                "knn_s": ctx.timing_read("knn")[0] / 1e3,
+                "pass1_s": ctx.timing_read("vecchia_pass1")[0] / 1e3,
+                "predict_s": ctx.timing_read("predict_blocks")[0] / 1e3}
+        (g6, mu6, sd6), (gm, mum, sdm) = fits["fp64"], fits["mixed"]
+        out["mixed_fit_vs_fp64_rel_err"] = {
+            "loglik": abs(gm.loglik_ - g6.loglik_) / abs(g6.loglik_),
+            "theta": rel(gm.theta_, g6.theta_), "pred_mean": rel(mum, mu6),
+            "pred_sd": rel(sdm, sd6), "test_points": len(Xt)}
     # CPU baseline: the C restatement of the oracle (OpenMP over blocks, as the
     # reference's prange), full C2 structure, repeated for ~cpu-seconds
     if not args.no_cpu:

@@ -17,6 +17,15 @@
 //   ll_reduce2 (1 workgroup): fixed-order sum, failed-chunk count.
 // All reductions are deterministic.
 //
+// In the split-row kernel (K <= 32) two phases are small matrix products on
+// the matrix cores (16x16x4 MFMA): the derivative right-hand sides dS_p a,
+// written as G [a | a x~ | a x~^2] with coordinates centred on the block's
+// own point, and the cross products W^T [W | L^-1 y | L^-1 Z] behind the
+// Fisher information and the gradient records.  FP64 mode runs them as FP64
+// MFMA; mixed mode (SVH_MIXED) as FP32 MFMA with FP32 accumulation.  The
+// covariance, Cholesky, log-determinant, triangular solves and GLS are FP64
+// in both modes, so the log-likelihood and beta do not depend on the mode.
+//
 // Cached record per block (need_grad):  u, v[q], log L_KK, wK[P], s[P], T[P*q]
 // where u = (L^-1 y)_K, v = (L^-1 Z)_K, W_p = L^-1 dS_p a, s_p = W_p.(L^-1 y),
 // T_pc = W_p.(L^-1 Z_c).  Then z = L^-1 y - (L^-1 Z) beta gives
@@ -363,6 +372,33 @@ __device__ __forceinline__ void matern_t(double r, double& M, double& g, int nu_
   }
 }
 
+// 16x16x4 matrix-core products.  Operands: lane L carries A[L & 15][L >> 4]
+// and B[L >> 4][L & 15].  FP64 (v_mfma_f64_16x16x4f64): result register v of
+// lane L is D[(L >> 4) + 4v][L & 15].  FP32 (v_mfma_f32_16x16x4f32, the mixed
+// mode): D[4 (L >> 4) + v][L & 15].
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+template <bool MIX>
+struct Mfma;
+template <>
+struct Mfma<false> {
+  typedef dvec4 acc_t;
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+  static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Mfma<true> {
+  typedef fvec4 acc_t;
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32((float)x, (float)y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+
 // LDS doubles per wave: Cb 32 | Bc 24 | Misc 8 | Lc 32*32 | Xs 32*d | Mt 32*33
 __host__ __device__ inline int split_lds_doubles(int d) {
   return 32 + 24 + 8 + 32 * 32 + 32 * d + 32 * 33;
@@ -371,18 +407,20 @@ __host__ __device__ inline int split_lds_doubles(int d) {
 // One block of the split-row kernel (body shared by the single-structure and
 // the batched kernels).  Accumulates this block's Fisher / GLS contributions
 // into the lane's task accumulators and writes the block record.
-template <int NU>
-#define SVH_PHASE(k)                                   \
-  if (a.prof) {                                        \
+// PROF adds per-phase shader-clock sums (s_memtime) for tuning runs.
+#define SVH_PHASE(k)                                            \
+  if constexpr (PROF) {                                         \
     const unsigned long long t_ = __builtin_amdgcn_s_memtime(); \
-    ph[k] += t_ - ph_t;                                \
-    ph_t = t_;                                         \
+    ph[k] += t_ - ph_t;                                         \
+    ph_t = t_;                                                  \
   }
 
+template <int NU, bool PROF, bool MIX>
 __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* lds,
-                                            double (&accF)[2], double (&accX)[3],
+                                            double (&accF)[4], double (&accX)[3],
                                             unsigned long long (&ph)[8]) {
-  unsigned long long ph_t = a.prof ? __builtin_amdgcn_s_memtime() : 0ull;
+  unsigned long long ph_t = 0ull;
+  if constexpr (PROF) ph_t = __builtin_amdgcn_s_memtime();
   const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
@@ -392,8 +430,6 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   double* Lc = Misc + 8;     // L' column-major: Lc[j*32 + i] = L'_ij (i > j)
   double* Xs = Lc + 32 * 32; // 32 x d scaled coordinates
   double* Mt = Xs + 32 * d;  // 32 x 33 scratch matrix
-  const int nF = grad ? P * (P + 1) / 2 : 0;
-  const int nT = grad ? P + P * q : 0;
   const int nX = q * q + q;
   const int dlo = h ? (d + 1) / 2 : 0, dhi = h ? d : (d + 1) / 2;   // this half's dims
   const int K = a.kcnt[b] + 1;
@@ -504,31 +540,65 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
 #pragma unroll
   for (int c = 0; c < kSC; ++c) R[c] = 0.0;
   if (grad) {
-    // dS_{1+l} a: half h sums j in [16h, 16h+16) for every dim, the halves
-    // are added, and half 1 keeps the result
-    double accl[kSD];
+    // dS_{1+l} a = -var sum_j g_rj a_j (x_rl - x_jl)^2.  With coordinates
+    // centred on the block's own point (x~ = x - x_{K-1}) this is
+    //   -var [x~_rl^2 E_r0 - 2 x~_rl E_r(1+l) + E_r(1+d+l)],  E = G B,
+    // B = [a | a x~_l | a x~_l^2]: a 32 x 32 x (1 + 2d) product on the
+    // matrix cores (2 x 2 output tiles, 8 steps of 4).
+    typedef Mfma<MIX> MF;
+    const int li = lane & 15, lk = lane >> 4;
+    const double* xK = Xs + (K - 1) * d;
+    const int NB = 1 + 2 * d;
+    // row tiles one at a time (16 accumulator registers live, not 32); the
+    // products go to registers first and to Mt only after every g is read
+    typename MF::acc_t Et[2][2];
 #pragma unroll
-    for (int l = 0; l < kSD; ++l) accl[l] = 0.0;
-#pragma unroll 4
-    for (int jj = 0; jj < kSH; ++jj) {
-      const int j = kSH * h + jj;
-      const double* gp = (j < r) ? &Mt[j * 33 + r] : &Mt[r * 33 + j];
-      const double gj = (j == r) ? 0.0 : *gp;
-      const double wj = gj * Cb[j];
-      const double* xj = Xs + j * d;
+    for (int rt = 0; rt < 2; ++rt) {
+      typename MF::acc_t A0 = MF::zero(), A1 = MF::zero();
+#pragma unroll 1
+      for (int st = 0; st < 8; ++st) {
+        const int j = 4 * st + lk;
+        const int rw = 16 * rt + li;
+        const double gv = (j == rw) ? 0.0 : (j < rw ? Mt[j * 33 + rw] : Mt[rw * 33 + j]);
+        const double aj = Cb[j];
+        double b0, b1;
+        {
+          const int c = li;   // tile 0: a, a x~_l, a x~_l^2 ...
+          const int l = (c <= d) ? c - 1 : c - 1 - d;
+          const double xt = (c >= 1 && c <= 2 * d) ? Xs[j * d + l] - xK[l] : 0.0;
+          b0 = (c == 0) ? aj : (c <= d ? aj * xt : (c <= 2 * d ? aj * xt * xt : 0.0));
+        }
+        {
+          const int c = 16 + li;
+          const int l = (c <= d) ? c - 1 : c - 1 - d;
+          const double xt = (c <= 2 * d) ? Xs[j * d + l] - xK[l] : 0.0;
+          b1 = (c <= d) ? aj * xt : (c <= 2 * d ? aj * xt * xt : 0.0);
+        }
+        A0 = MF::mma(gv, b0, A0);
+        A1 = MF::mma(gv, b1, A1);
+      }
+      Et[rt][0] = A0;
+      Et[rt][1] = A1;
+    }
+    __syncthreads();   // every read of g is done: E goes to Mt, column-major
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = 16 * rt + MF::row(lk, v);
+        if (li < NB) Mt[li * 33 + rr] = (double)Et[rt][0][v];
+        if (16 + li < NB) Mt[(16 + li) * 33 + rr] = (double)Et[rt][1][v];
+      }
+    __syncthreads();
+    if (h == 1) {
+      const double e0 = Mt[r];
 #pragma unroll
       for (int l = 0; l < kSD; ++l)
         if (l < d) {
-          const double dd = xr[l] - xj[l];
-          accl[l] = fma(wj * dd, dd, accl[l]);
+          const double xt = Xs[r * d + l] - xK[l];
+          const double v = xt * (xt * e0 - 2.0 * Mt[(1 + l) * 33 + r]) + Mt[(1 + d + l) * 33 + r];
+          R[1 + l] = -a.var * v;
         }
-    }
-#pragma unroll
-    for (int l = 0; l < kSD; ++l) {
-      const double full = accl[l] + __shfl_xor(accl[l], 32, 64);
-      if (l < d && h == 1) R[1 + l] = -a.var * full;
-    }
-    if (h == 1) {
       R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
       R[kSC - 1] = a.var * a.nug * av;
     }
@@ -592,32 +662,38 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
       v = Mt[(e - q - 2) * 33 + km];
     if (ok) rec[e] = v;
   }
-  for (int e = lane; e < nF + nT; e += kWave) {
-    int c1, c2;
-    if (e < nF) {
-      pair_of(e, c1, c2);
-    } else if (e < nF + P) {
-      c1 = e - nF;
-      c2 = cyl;
-    } else {
-      const int rr = e - nF - P;
-      c1 = rr / q;
-      c2 = cyl + 1 + (rr - c1 * q);
+  if (grad) {
+    // D = W^T [W | yt | Zt] on the matrix cores: one 16-row tile (p < P),
+    // two 16-column tiles, 8 steps of 4 rows.  Column c of the scratch is
+    // both A row c and B column c.
+    typedef Mfma<MIX> MF;
+    const int li = lane & 15, lk = lane >> 4;
+    typename MF::acc_t D0 = MF::zero(), D1 = MF::zero();
+#pragma unroll 2
+    for (int st = 0; st < 8; ++st) {
+      const int i = 4 * st + lk;
+      const double w0 = Mt[li * 33 + i];
+      const double w1 = Mt[(16 + li) * 33 + i];
+      D0 = MF::mma(w0, w0, D0);
+      D1 = MF::mma(w0, w1, D1);
     }
-    double sdot = 0.0;
-    for (int i = 0; i < K; ++i) sdot = fma(Mt[c1 * 33 + i], Mt[c2 * 33 + i], sdot);
-    if (e < nF) {
-      const double f = sdot - 0.5 * Mt[c1 * 33 + km] * Mt[c2 * 33 + km];
-      if (ok) {
-        if (e < kWave)
-          accF[0] += f;
-        else
-          accF[1] += f;
+    const int N = cyl + 1 + q;
+    const double wKc0 = Mt[li * 33 + km];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int p = MF::row(lk, v);
+      if (p < P) {
+        const double d0 = (double)D0[v], d1 = (double)D1[v];
+        if (li < P) {
+          const double f = d0 - 0.5 * Mt[p * 33 + km] * wKc0;
+          if (ok) accF[v] += f;
+        } else if (li == cyl) {
+          if (ok) rec[2 + q + P + p] = d0;
+        } else if (li < N) {
+          if (ok) rec[2 + q + 2 * P + p * q + (li - cyl - 1)] = d0;
+        }
+        if (16 + li < N && ok) rec[2 + q + 2 * P + p * q + (16 + li - cyl - 1)] = d1;
       }
-    } else if (e < nF + P) {
-      if (ok) rec[2 + q + P + (e - nF)] = sdot;
-    } else {
-      if (ok) rec[2 + q + 2 * P + (e - nF - P)] = sdot;
     }
   }
   for (int e = lane; e < nX; e += kWave) {
@@ -641,39 +717,50 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   SVH_PHASE(6)
 }
 
-template <int NU>
-__global__ __launch_bounds__(kWave) void ll_pass1_split(LLArgs a) {
+// Fisher partial sums of one workgroup: lane L accumulated entries
+// (Mfma row(L >> 4, v), L & 15) of the P x P matrix.
+template <bool MIX>
+__device__ __forceinline__ void write_fisher(double* part, const double (&accF)[4], int P,
+                                             int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int p = Mfma<MIX>::row(lk, v);
+    if (p < P && li < P) part[p * P + li] = accF[v];
+  }
+}
+
+template <int NU, bool PROF, bool MIX>
+__global__ __launch_bounds__(kWave, 2) void ll_pass1_split(LLArgs a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
   const int P = a.P, q = a.q;
   const bool grad = a.need_grad != 0;
-  const int nF = grad ? P * (P + 1) / 2 : 0;
   const int nX = q * q + q;
-  double accF[2] = {0.0, 0.0};   // Fisher tasks lane, lane+64 (nF <= 78)
+  double accF[4] = {0.0, 0.0, 0.0, 0.0};   // Fisher entries of this lane
   double accX[3] = {0.0, 0.0, 0.0};  // GLS tasks lane, +64, +128 (nX <= 132)
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x)
-    split_block<NU>(a, b, lds, accF, accX, ph);
-  if (a.prof && lane == 0)
+    split_block<NU, PROF, MIX>(a, b, lds, accF, accX, ph);
+  if (PROF && lane == 0)
     for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   // this workgroup's partials (layout of ll_pass1: F (P*P), XtX (q*q), Zty (q))
   const int NACC = grad ? P * P + q * q + q : q * q + q;
   const int X0 = grad ? P * P : 0;
   double* part = a.partial + (int64_t)blockIdx.x * NACC;
-  for (int e = lane; e < nF; e += kWave) {
-    int c1, c2;
-    pair_of(e, c1, c2);
-    const double f = (e < kWave) ? accF[0] : accF[1];
-    part[c1 * P + c2] = f;
-    part[c2 * P + c1] = f;
-  }
+  if (grad) write_fisher<MIX>(part, accF, P, lane);
   for (int e = lane; e < nX; e += kWave)
     part[X0 + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
 }
 
 template <int NU>
-void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a) {
-  hipLaunchKernelGGL((ll_pass1_split<NU>), dim3(G), dim3(kWave), lds, st, a);
+void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed) {
+  if (a.prof)
+    hipLaunchKernelGGL((ll_pass1_split<NU, true, false>), dim3(G), dim3(kWave), lds, st, a);
+  else if (mixed)
+    hipLaunchKernelGGL((ll_pass1_split<NU, false, true>), dim3(G), dim3(kWave), lds, st, a);
+  else
+    hipLaunchKernelGGL((ll_pass1_split<NU, false, false>), dim3(G), dim3(kWave), lds, st, a);
 }
 
 // ---------------------------------------------------------------- reduce 1
@@ -925,7 +1012,7 @@ struct BatchArgs {
   int mmax_stride, GX;
 };
 
-__global__ __launch_bounds__(kWave) void ll_pass1_batch(BatchArgs B) {
+__global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
   extern __shared__ double lds[];
   const int i = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
   if (!fs_evaluates(B.fs[i].status)) return;
@@ -954,22 +1041,16 @@ __global__ __launch_bounds__(kWave) void ll_pass1_batch(BatchArgs B) {
   a.var = B.par[i * 16 + 0];
   a.nug = B.par[i * 16 + 1];
   a.rec_stride = kBNC;
-  double accF[2] = {0.0, 0.0};
+  double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   a.prof = nullptr;
   for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
-    split_block<-1>(a, b, lds, accF, accX, ph);
+    split_block<-1, false, false>(a, b, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
-  const int nF = P * (P + 1) / 2, nX = q * q + q;
+  const int nX = q * q + q;
   double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
-  for (int e = lane; e < nF; e += kWave) {
-    int c1, c2;
-    pair_of(e, c1, c2);
-    const double f = (e < kWave) ? accF[0] : accF[1];
-    part[c1 * P + c2] = f;
-    part[c2 * P + c1] = f;
-  }
+  write_fisher<false>(part, accF, P, lane);
   for (int e = lane; e < nX; e += kWave)
     part[P * P + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
 }
@@ -1455,9 +1536,9 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     ctx->timers.begin(s, "vecchia_pass1");
     if (use_split) {
       switch (nu_code) {
-        case 0: launch_split_nu<0>(s, G, lds_split, a); break;
-        case 1: launch_split_nu<1>(s, G, lds_split, a); break;
-        default: launch_split_nu<2>(s, G, lds_split, a); break;
+        case 0: launch_split_nu<0>(s, G, lds_split, a, precision == SVH_MIXED); break;
+        case 1: launch_split_nu<1>(s, G, lds_split, a, precision == SVH_MIXED); break;
+        default: launch_split_nu<2>(s, G, lds_split, a, precision == SVH_MIXED); break;
       }
     } else {
       switch (KP) {
@@ -1510,7 +1591,6 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     std::fprintf(stderr, "[svh phase clocks] gather %llu pairs %llu chol %llu back %llu rhs %llu "
                  "solve %llu dots %llu\n", hp[0], hp[1], hp[2], hp[3], hp[4], hp[5], hp[6]);
   }
-  (void)precision;
 }
 
 // ---------------------------------------------------------------- batched host side

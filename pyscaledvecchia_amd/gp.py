@@ -41,7 +41,10 @@ class ScaledVecchiaGP:
     nugget         : None -> estimated relative nugget; float -> fixed
     var_correction : estimate the predictive-variance factor b (Sec. 3.4)
     lambda_max     : ranges above this are treated as infinite
-    precision      : 'fp64' (default) or 'mixed' likelihood/prediction kernels
+    precision      : 'fp64' (default) or 'mixed': FP32 matrix-core products for
+                     the derivative terms and Fisher information during the
+                     search (log-likelihood, Cholesky, GLS stay FP64), then
+                     FP64 refinement steps
     theta0         : optional initial log-parameters for Fisher scoring
     """
 
@@ -210,7 +213,7 @@ class ScaledVecchiaGP:
         last_ranges = np.exp(theta[1:1 + d]).copy()
         log_var_target = prep["lvt"]
 
-        def objective(th):
+        def objective(th, precision=self.precision):
             th = self._clip_theta(np.array(th, dtype=float), d)
             if fixed_nug:
                 th[-1] = math.log(self.nugget)
@@ -218,7 +221,7 @@ class ScaledVecchiaGP:
             ll, g, M, beta = state["struct"](th, self.nu, need_grad=True,
                                              var_penalty=self.var_penalty,
                                              log_var_target=log_var_target,
-                                             precision=self.precision)
+                                             precision=precision)
             if fixed_nug:
                 g = g.copy(); g[-1] = 0.0
                 M = M.copy(); M[-1, :] = 0.0; M[:, -1] = 0.0; M[-1, -1] = 1.0
@@ -239,6 +242,18 @@ class ScaledVecchiaGP:
         theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
                                           tol=self.tol, verbose=self.verbose,
                                           reorder_hook=reorder_hook, history=self.history_)
+        self.n_refine_ = 0
+        if self.precision == "mixed":
+            # Refinement: the mixed search used FP64 log-likelihoods but FP32
+            # derivative products, so its stationary point is that of a
+            # slightly perturbed gradient.  Fisher scoring with FP64
+            # derivatives on the final structure moves it to the FP64 one
+            # (one or two steps from this start).
+            n0 = len(self.history_)
+            theta, ll, beta = _fisher_scoring(
+                theta, lambda th: objective(th, "fp64"), max_iter=self.max_iter,
+                tol=self.tol, verbose=self.verbose, reorder_hook=None, history=self.history_)
+            self.n_refine_ = len(self.history_) - n0
         self._fit_state = state
         return self._finish(theta, ll, beta, rng, n)
 
