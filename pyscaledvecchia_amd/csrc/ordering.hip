@@ -120,18 +120,22 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 //             unselected point (s2 = min squared distance, dist = sqrt(s2));
 //             each 256-point slice publishes its top-(kSliceTop+1) points by
 //             (dist desc, index asc).
-//   mm_select (1 wave): V = the largest published "next" element over all
-//             slices is an upper bound (in that order) on every point that is
-//             not a published candidate.  Candidates above V form a pool;
-//             selection repeats  "take the pool maximum, lower pool distances
-//             with the new point"  while the maximum stays above V (the first
-//             pick of a round is the global maximum and always valid).  Each
-//             pick is exactly the serial algorithm's next step: every point
-//             outside the pool is at or below V and distances only fall.
+//   mm_select (one workgroup, one pool slot per thread): V = the largest
+//             published "next" element over all slices is an upper bound (in
+//             that order) on every point that is not a published candidate.
+//             Candidates above V form a pool; selection repeats "take the pool
+//             maximum, lower pool distances with the new point" while the
+//             maximum stays above V (the first pick of a round is the global
+//             maximum and always valid).  Each pick is exactly the serial
+//             algorithm's next step: every point outside the pool is at or
+//             below V and distances only fall.
+// Coordinates are held in DM >= d registers per point with the extra
+// coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
+// the squared distances are bit-identical to the d-term sums.
 constexpr int kSlice = 256;
 constexpr int kSliceTop = 4;          // published candidates per slice
-constexpr int kPool = 256;            // pool entries (4 per lane)
-constexpr int kAccCap = 256;          // max accepted points per round
+constexpr int kPool = 256;            // pool slots = select threads
+constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
 constexpr int kMMD = 12;              // max d of the maximin kernels
 
@@ -149,12 +153,44 @@ struct MMDesc {
   int n, d, first, slice0, nslice, pad;
 };
 
+// Wave arg-max of (key desc, idx asc), exact.  Fast path: one DPP max over
+// the high words of the order-preserving keys and a ballot; the low words and
+// the index are compared only when several lanes share the maximal high word.
+// Returns the winner's key, idx and `e`.  Every lane must be active.
+__device__ __forceinline__ void wave_argmax_fast(double k, int i, int e, double& bk, int& bi,
+                                                 int& be) {
+  const unsigned long long u = ord_key(k);
+  const unsigned hi = (unsigned)(u >> 32), lo = (unsigned)u;
+  const unsigned m1 = dpp_wave_max(hi);
+  unsigned long long ball = __ballot(hi == m1);
+  if (__popcll(ball) != 1) {
+    const unsigned m2 = dpp_wave_max(hi == m1 ? lo : 0u);
+    const bool tie = (hi == m1) && (lo == m2);
+    ball = __ballot(tie);
+    if (__popcll(ball) != 1) {
+      const unsigned mi = dpp_wave_max(tie ? ~(unsigned)i : 0u);
+      ball = __ballot(tie && i == (int)~mi);
+    }
+  }
+  const int wl = (int)__ffsll((long long)ball) - 1;
+  const long long kb = __double_as_longlong(k);
+  const int klo = __builtin_amdgcn_readlane((int)(kb & 0xffffffffll), wl);
+  const int khi = __builtin_amdgcn_readlane((int)(kb >> 32), wl);
+  bk = __longlong_as_double(((long long)khi << 32) | (unsigned)klo);
+  bi = __builtin_amdgcn_readlane(i, wl);
+  be = __builtin_amdgcn_readlane(e, wl);
+}
+
+template <int DM>
 __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     double* __restrict__ s2all, double* __restrict__ distall, const int* __restrict__ accall,
     const MMState* __restrict__ stall, double* __restrict__ cand_keyall,
     int* __restrict__ cand_idxall) {
-  extern __shared__ double smem[];
+  __shared__ double accX[kAccCap * DM];
+  __shared__ int accI[kAccCap];
+  __shared__ double wk[kSlice / kWave];
+  __shared__ int wi[kSlice / kWave];
   const MMDesc D = desc[blockIdx.y];
   if ((int)blockIdx.x >= D.nslice) return;
   const int n = D.n, d = D.d;
@@ -167,15 +203,11 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
   double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
   int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
   const int A = st->nacc;
-  double* accX = smem;  // A * d
-  __shared__ double wk[kSlice / kWave];
-  __shared__ int wi[kSlice / kWave];
-  __shared__ int accI[kAccCap];
-  for (int t = threadIdx.x; t < A; t += blockDim.x) accI[t] = acc[t];
+  for (int t = threadIdx.x; t < A; t += kSlice) accI[t] = acc[t];
   __syncthreads();
-  for (int t = threadIdx.x; t < A * d; t += blockDim.x) {
-    int a = t / d, l = t - a * d;
-    accX[t] = XT[(int64_t)l * n + accI[a]];
+  for (int t = threadIdx.x; t < A * DM; t += kSlice) {
+    const int a = t / DM, l = t - a * DM;
+    accX[t] = (l < d) ? XT[(int64_t)l * n + accI[a]] : 0.0;
   }
   __syncthreads();
   const int i = blockIdx.x * kSlice + threadIdx.x;
@@ -183,16 +215,18 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
   if (i < n) {
     di = dist[i];
     if (di >= 0.0 && A > 0) {
+      double xi[DM];
+#pragma unroll
+      for (int l = 0; l < DM; ++l) xi[l] = (l < d) ? XT[(int64_t)l * n + i] : 0.0;
       double s = s2[i];
       bool changed = false, picked = false;
+#pragma unroll 2
       for (int a = 0; a < A; ++a) {
-        if (accI[a] == i) {
-          picked = true;
-          break;
-        }
+        picked |= (accI[a] == i);
         double t = 0.0;
-        const double* xa = accX + a * d;
-        for (int l = 0; l < d; ++l) t = sq_step(t, XT[(int64_t)l * n + i], xa[l]);
+        const double* xa = accX + a * DM;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) t = sq_step(t, xi[l], xa[l]);
         if (t < s) {
           s = t;
           changed = true;
@@ -240,11 +274,35 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
   }
 }
 
-__global__ __launch_bounds__(kWave) void mm_select_kernel(
+// Workgroup arg-max of (key desc, idx asc); every thread gets the result.
+template <int NT>
+__device__ __forceinline__ void block_argmax(double& k, int& i, double* rk, int* ri) {
+  constexpr int NW = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  wave_argmax(k, i);
+  __syncthreads();
+  if (lane == 0) {
+    rk[wid] = k;
+    ri[wid] = i;
+  }
+  __syncthreads();
+  k = rk[0];
+  i = ri[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w)
+    if (mm_greater(rk[w], ri[w], k, i)) {
+      k = rk[w];
+      i = ri[w];
+    }
+}
+
+template <int DM>
+__global__ __launch_bounds__(kPool) void mm_select_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     const double* __restrict__ s2all, const double* __restrict__ cand_keyall,
     const int* __restrict__ cand_idxall, int* __restrict__ accall, MMState* __restrict__ stall,
     int64_t* __restrict__ orderall, int sortcap) {
+  constexpr int NT = kPool, NW = NT / kWave;
   const MMDesc D = desc[blockIdx.x];
   const int n = D.n, d = D.d, nslice = D.nslice;
   const double* XT = XTall + D.xoff;
@@ -254,47 +312,62 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
   int* acc = accall + (int64_t)blockIdx.x * kAccCap;
   MMState* st = stall + blockIdx.x;
   int64_t* order = orderall + D.soff;
-  // LDS: survivors (sortcap keys + sortcap indices)
+  // LDS: survivors (sortcap keys + sortcap indices), pool coordinates,
+  // per-wave reduction slots (double-buffered over picks)
   extern __shared__ double smem[];
   double* sk = smem;
   int* si = reinterpret_cast<int*>(smem + sortcap);
-  const int lane = threadIdx.x;
+  __shared__ double pc[NT * DM];
+  __shared__ double rk[NW];
+  __shared__ int ri[NW], wcnt[NW];
+  __shared__ double bks[2][NW];
+  __shared__ int bis[2][NW], bes[2][NW];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
   if (count0 >= n) {
-    if (lane == 0) st->nacc = 0;
+    if (tid == 0) st->nacc = 0;
     return;
   }
   // bound V: the largest "next" element over all slices
   double Vk = -3.0;
   int Vi = INT_MAX;
-  for (int s = lane; s < nslice; s += kWave) {
-    double k = cand_key[s * (kSliceTop + 1) + kSliceTop];
-    int i = cand_idx[s * (kSliceTop + 1) + kSliceTop];
+  for (int s = tid; s < nslice; s += NT) {
+    const double k = cand_key[s * (kSliceTop + 1) + kSliceTop];
+    const int i = cand_idx[s * (kSliceTop + 1) + kSliceTop];
     if (mm_greater(k, i, Vk, Vi)) {
       Vk = k;
       Vi = i;
     }
   }
-  wave_argmax(Vk, Vi);
+  block_argmax<NT>(Vk, Vi, rk, ri);
   // survivors: published candidates strictly above V, compacted into LDS.
   // Survivors that do not fit raise the bound instead (they leave the pool).
   int cnt = 0;
   double Ok = -3.0;
   int Oi = INT_MAX;
   const int ncand = nslice * kSliceTop;
-  for (int base = 0; base < ncand; base += kWave) {
-    int c = base + lane;
+  for (int base = 0; base < ncand; base += NT) {
+    const int c = base + tid;
     double k = -3.0;
     int i = INT_MAX;
     bool keep = false;
     if (c < ncand) {
-      int s = c / kSliceTop, r = c - s * kSliceTop;
+      const int s = c / kSliceTop, r = c - s * kSliceTop;
       k = cand_key[s * (kSliceTop + 1) + r];
       i = cand_idx[s * (kSliceTop + 1) + r];
       keep = k >= 0.0 && mm_greater(k, i, Vk, Vi);
     }
-    unsigned long long m = __ballot(keep);
-    int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+    const unsigned long long m = __ballot(keep);
+    __syncthreads();
+    if (lane == 0) wcnt[wid] = __popcll(m);
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      woff += (w < wid) ? wcnt[w] : 0;
+      tot += wcnt[w];
+    }
+    const int pos = cnt + woff + __popcll(m & ((1ull << lane) - 1ull));
     if (keep) {
       if (pos < sortcap) {
         sk[pos] = k;
@@ -304,33 +377,33 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
         Oi = i;
       }
     }
-    cnt += __popcll(m);
+    cnt += tot;
   }
-  wave_argmax(Ok, Oi);
+  block_argmax<NT>(Ok, Oi, rk, ri);
   if (mm_greater(Ok, Oi, Vk, Vi)) {
     Vk = Ok;
     Vi = Oi;
   }
   int nsurv = cnt < sortcap ? cnt : sortcap;
   __syncthreads();
-  if (nsurv > kPool) {
-    // keep the top kPool survivors: bitonic sort (descending) in LDS
+  if (nsurv > NT) {
+    // keep the top NT survivors: bitonic sort (descending) in LDS
     int L = 1;
     while (L < nsurv) L <<= 1;
-    for (int t = nsurv + lane; t < L; t += kWave) {
+    for (int t = nsurv + tid; t < L; t += NT) {
       sk[t] = -3.0;
       si[t] = INT_MAX;
     }
     __syncthreads();
     for (int size = 2; size <= L; size <<= 1) {
       for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int t = lane; t < L / 2; t += kWave) {
-          int lo = 2 * stride * (t / stride) + (t % stride);
-          int hi = lo + stride;
-          bool desc = (lo & size) == 0;
-          double ka = sk[lo], kb = sk[hi];
-          int ia = si[lo], ib = si[hi];
-          bool sw = desc ? mm_greater(kb, ib, ka, ia) : mm_greater(ka, ia, kb, ib);
+        for (int t = tid; t < L / 2; t += NT) {
+          const int lo = 2 * stride * (t / stride) + (t % stride);
+          const int hi = lo + stride;
+          const bool desc_ = (lo & size) == 0;
+          const double ka = sk[lo], kb = sk[hi];
+          const int ia = si[lo], ib = si[hi];
+          const bool sw = desc_ ? mm_greater(kb, ib, ka, ia) : mm_greater(ka, ia, kb, ib);
           if (sw) {
             sk[lo] = kb;
             sk[hi] = ka;
@@ -342,93 +415,75 @@ __global__ __launch_bounds__(kWave) void mm_select_kernel(
       }
     }
     // tighten the bound with the first entry left out
-    double k = sk[kPool];
-    int i = si[kPool];
+    const double k = sk[NT];
+    const int i = si[NT];
     if (mm_greater(k, i, Vk, Vi)) {
       Vk = k;
       Vi = i;
     }
-    nsurv = kPool;
-    if (lane == 0) st->overflow_sorts += 1;
+    nsurv = NT;
+    if (tid == 0) st->overflow_sorts += 1;
     __syncthreads();
   }
-  // load pool: kPool / 64 slots per lane, coordinates in registers
-  constexpr int kNT = kPool / kWave;
-  double pv[kNT], ps[kNT], px[kNT][kMMD];
-  int pi[kNT];
-#pragma unroll
-  for (int t = 0; t < kNT; ++t) {
-    int e = lane + t * kWave;
-    if (e < nsurv) {
-      pv[t] = sk[e];
-      pi[t] = si[e];
-      ps[t] = s2g[pi[t]];
-    } else {
-      pv[t] = -3.0;
-      pi[t] = INT_MAX;
-      ps[t] = 0.0;
-    }
-#pragma unroll
-    for (int l = 0; l < kMMD; ++l)
-      px[t][l] = (l < d && e < nsurv) ? XT[(int64_t)l * n + pi[t]] : 0.0;
+  // pool: one slot per thread, coordinates in registers and in LDS
+  double pv = -3.0, ps = 0.0, px[DM];
+  int pi = INT_MAX;
+  if (tid < nsurv) {
+    pv = sk[tid];
+    pi = si[tid];
+    ps = s2g[pi];
   }
-  int nacc = 0;
+#pragma unroll
+  for (int l = 0; l < DM; ++l) {
+    px[l] = (l < d && tid < nsurv) ? XT[(int64_t)l * n + pi] : 0.0;
+    pc[tid * DM + l] = px[l];
+  }
+  __syncthreads();
+  int nacc = 0, par = 0;
   const int room = n - count0;
   while (nacc < kAccCap && nacc < room) {
-    // this lane's best slot, with its coordinates
-    double lk = -3.0, lx[kMMD];
-    int li = INT_MAX;
-#pragma unroll
-    for (int l = 0; l < kMMD; ++l) lx[l] = 0.0;
-#pragma unroll
-    for (int t = 0; t < kNT; ++t) {
-      const bool b = mm_greater(pv[t], pi[t], lk, li);
-      lk = b ? pv[t] : lk;
-      li = b ? pi[t] : li;
-#pragma unroll
-      for (int l = 0; l < kMMD; ++l) lx[l] = b ? px[t][l] : lx[l];
-    }
-    double bk;
-    int bi, wl;
-    wave_argmax_dpp(lk, li, lane, bk, bi, wl);
-    if (bk < 0.0) break;
-    if (nacc > 0 && !mm_greater(bk, bi, Vk, Vi)) break;
+    double wk;
+    int wi, we;
+    wave_argmax_fast(pv, pi, tid, wk, wi, we);
     if (lane == 0) {
-      acc[nacc] = bi;
-      order[count0 + nacc] = bi;
+      bks[par][wid] = wk;
+      bis[par][wid] = wi;
+      bes[par][wid] = we;
+    }
+    __syncthreads();
+    double gk = bks[par][0];
+    int gi = bis[par][0], ge = bes[par][0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w)
+      if (mm_greater(bks[par][w], bis[par][w], gk, gi)) {
+        gk = bks[par][w];
+        gi = bis[par][w];
+        ge = bes[par][w];
+      }
+    par ^= 1;
+    if (gk < 0.0) break;
+    if (nacc > 0 && !mm_greater(gk, gi, Vk, Vi)) break;
+    if (tid == 0) {
+      acc[nacc] = gi;
+      order[count0 + nacc] = gi;
     }
     ++nacc;
-    // coordinates of the new point from the winning lane (scalar registers)
-    double sx[kMMD];
-#pragma unroll
-    for (int l = 0; l < kMMD; ++l) {
-      const long long v = __double_as_longlong(lx[l]);
-      const int vlo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), wl);
-      const int vhi = __builtin_amdgcn_readlane((int)(v >> 32), wl);
-      sx[l] = __longlong_as_double(((long long)vhi << 32) | (unsigned)vlo);
-    }
-#pragma unroll
-    for (int t = 0; t < kNT; ++t) {
-      if (pi[t] == bi) {
-        pv[t] = -3.0;
-        continue;
-      }
+    if (pi == gi) {
+      pv = -3.0;
+    } else if (pv >= 0.0 && mm_greater(pv, pi, Vk, Vi)) {
       // only entries still above V can be picked later in this round
-      if (pv[t] < 0.0 || !mm_greater(pv[t], pi[t], Vk, Vi)) {
-        pv[t] = -3.0;
-        continue;
-      }
       double s = 0.0;
 #pragma unroll
-      for (int l = 0; l < kMMD; ++l)
-        if (l < d) s = sq_step(s, px[t][l], sx[l]);
-      if (s < ps[t]) {
-        ps[t] = s;
-        pv[t] = sqrt(s);
+      for (int l = 0; l < DM; ++l) s = sq_step(s, px[l], pc[ge * DM + l]);
+      if (s < ps) {
+        ps = s;
+        pv = sqrt(s);
       }
+    } else {
+      pv = -3.0;
     }
   }
-  if (lane == 0) {
+  if (tid == 0) {
     st->nacc = nacc;
     st->count = count0 + nacc;
     st->rounds += 1;
@@ -644,8 +699,9 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
                      s2, dist, acc, st, dorder);
   SVH_HIP(hipGetLastError());
   if (maxn <= 1) return;
-  const size_t apply_lds = (size_t)kAccCap * dmax * sizeof(double);
-  SVH_REQUIRE(apply_lds <= 64 * 1024 && dmax <= kMMD, "maximin: d above 12 not supported");
+  SVH_REQUIRE(dmax <= kMMD, "maximin: d above 12 not supported");
+  // coordinate registers per point: d rounded up to 4, 8 or 12 (zero padded)
+  const int dm = dmax <= 4 ? 4 : (dmax <= 8 ? 8 : 12);
   int sortcap = 1;
   while (sortcap < maxslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
@@ -662,12 +718,27 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   for (;;) {
     for (int r = 0; r < batch; ++r) {
       ctx->timers.begin(ctx->stream, "maximin_apply");
-      hipLaunchKernelGGL(mm_apply_kernel, dim3(maxslice, nd), dim3(kSlice), apply_lds, ctx->stream,
-                         dd, dXT, s2, dist, acc, st, ck, ci);
+      const dim3 ga(maxslice, nd);
+      if (dm == 4)
+        hipLaunchKernelGGL(mm_apply_kernel<4>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+                           acc, st, ck, ci);
+      else if (dm == 8)
+        hipLaunchKernelGGL(mm_apply_kernel<8>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+                           acc, st, ck, ci);
+      else
+        hipLaunchKernelGGL(mm_apply_kernel<12>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+                           acc, st, ck, ci);
       ctx->timers.end(ctx->stream);
       ctx->timers.begin(ctx->stream, "maximin_select");
-      hipLaunchKernelGGL(mm_select_kernel, dim3(nd), dim3(kWave), select_lds, ctx->stream, dd, dXT,
-                         s2, ck, ci, acc, st, dorder, sortcap);
+      if (dm == 4)
+        hipLaunchKernelGGL(mm_select_kernel<4>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
+      else if (dm == 8)
+        hipLaunchKernelGGL(mm_select_kernel<8>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
+      else
+        hipLaunchKernelGGL(mm_select_kernel<12>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
       ctx->timers.end(ctx->stream);
     }
     SVH_HIP(hipGetLastError());

@@ -45,6 +45,10 @@ def test_maximin_large_prefix_and_property(gpu):
     assert np.array_equal(np.sort(o), np.arange(n))
     ref = O.maximin_order(X, first, steps=1500)
     np.testing.assert_array_equal(o[:1500], ref)
+    # the whole order against the C restatement of the oracle (~20 s on one core)
+    from oracle import c_oracle as C
+    if C.available():
+        np.testing.assert_array_equal(o, C.maximin_order(X, first))
     # selected distances to the chosen set never increase
     nn = gpu.ordered_knn(X[o], 1)
     r = np.sqrt(((X[o][1:] - X[o][nn[1:, 1]]) ** 2).sum(1))
