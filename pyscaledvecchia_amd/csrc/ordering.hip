@@ -2,8 +2,8 @@
 // neighbour sets.
 //
 // Distances are the canonical squared distance: sequential sum over the d
-// coordinates of diff*diff, each product rounded before the add (explicit
-// __dmul_rn / __dadd_rn, so no FMA contraction).  Sets are taken by
+// coordinates of diff*diff, each product rounded before the add (FMA
+// contraction switched off in sq_step).  Sets are taken by
 // (squared distance, index); the maximin arg-max takes the lowest index among
 // equal sqrt-distances.  The results are therefore bit-identical to a CPU
 // evaluation of the same definitions.
@@ -17,9 +17,14 @@ namespace {
 
 constexpr int kWave = 64;
 
+// One term of the canonical squared distance: s + (a-b)^2 with the product
+// rounded before the add.  HIP's __dmul_rn/__dadd_rn are plain operators, so
+// contraction into an FMA is switched off here explicitly.
 __device__ __forceinline__ double sq_step(double s, double a, double b) {
-  double t = a - b;
-  return __dadd_rn(s, __dmul_rn(t, t));
+#pragma clang fp contract(off)
+  const double t = a - b;
+  const double p = t * t;
+  return s + p;
 }
 
 // (key, idx) with "larger" = larger key, then smaller idx.

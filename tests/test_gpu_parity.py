@@ -76,6 +76,19 @@ def test_knn_query_bit_exact(gpu, nref, nq, d, k):
     np.testing.assert_array_equal(gpu.knn_query(Xr, Xq, k), O.knn_query(Xr, Xq, k))
 
 
+def test_knn_distance_is_not_fma_contracted(gpu):
+    """Near-tie pairs whose nearest member flips if any squared-distance term
+    is fused into an FMA: the kernels must round each product (as the oracle)."""
+    from _helpers import near_tie_pairs
+    Xr, Xq = near_tie_pairs()
+    nb = gpu.knn_query(Xr, Xq, 1)
+    np.testing.assert_array_equal(nb[:, 0], 2 * np.arange(len(Xq)) + 1)
+    # ordered form and maximin share the distance routine: the same pairs as
+    # one ordered set (each query last in its group of three)
+    X = np.vstack([np.vstack([Xr[2 * k], Xr[2 * k + 1], Xq[k]]) for k in range(len(Xq))])
+    np.testing.assert_array_equal(gpu.ordered_knn(X, 1), O.find_ordered_nn(X, 1))
+
+
 # ------------------------------------------------------------------ (b)
 def _toy(n=60, d=3, seed=1):
     rng = np.random.default_rng(seed)

@@ -122,6 +122,16 @@ def test_knn_query_matches_sorted_distances():
         np.testing.assert_array_equal(nb[i], np.argsort(dist, kind="stable")[:12])
 
 
+def test_distance_rounding_is_sequential_without_fma():
+    """The canonical squared distance rounds each product before the add: on
+    near-tie pairs where either fused evaluation would pick the first
+    candidate, the oracle picks the second (pins the no-FMA definition)."""
+    from _helpers import near_tie_pairs
+    Xr, Xq = near_tie_pairs()
+    nb = O.knn_query(Xr, Xq, 1)
+    np.testing.assert_array_equal(nb[:, 0], 2 * np.arange(len(Xq)) + 1)
+
+
 # ---------------------------------------------------------------- blocks
 @pytest.mark.parametrize("nu", [0.5, 1.5, 2.5])
 def test_block_cov_derivatives_match_fd(nu):
