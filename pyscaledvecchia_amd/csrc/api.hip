@@ -18,7 +18,8 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
                            const double* dXt, const int64_t* dtg, int tg_stride,
                            const double* dmask, int noise_free, int64_t nblk, int K, int d,
                            const double* theta, double nu, const double* dresid, int mode,
-                           double* dout_a, double* dmean, double* dvar, int* jitter_power_max);
+                           double* dout_a, double* dmean, double* dvar, int* jitter_power_max,
+                           int precision);
 
 void batch_release(svh_ctx* ctx);
 void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t* d,
@@ -232,7 +233,7 @@ int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t nt
     launch_to_soa(ctx, dQ, nq, d, dlam, dQT);
     launch_knn_query(ctx, dRT, ntr, dQT, nq, d, m, dnb, m);
     predict_blocks_device(ctx, dR, dnb, m, dQ, nullptr, 0, nullptr, noise_free, nq, m + 1, d,
-                          theta, nu, dres, 1, nullptr, dmean, dvar, nullptr);
+                          theta, nu, dres, 1, nullptr, dmean, dvar, nullptr, precision);
     download(ctx, mean, dmean, (size_t)nq * sizeof(double));
     download(ctx, var, dvar, (size_t)nq * sizeof(double));
     if (nb_out) download(ctx, nb_out, dnb, (size_t)nq * m * sizeof(int64_t));
@@ -258,7 +259,7 @@ int svh_block_columns(svh_ctx* ctx, const double* Xall, int64_t nall, int32_t d,
     upload(ctx, dblk, blocks, (size_t)nblk * K * sizeof(int64_t));
     if (mask) upload(ctx, dmask, mask, (size_t)nblk * K * sizeof(double));
     predict_blocks_device(ctx, dX, dblk, K, dX, dblk + (K - 1), K, dmask, 0, nblk, K, d, theta, nu,
-                          nullptr, 0, da, nullptr, nullptr, nullptr);
+                          nullptr, 0, da, nullptr, nullptr, nullptr, precision);
     download(ctx, a, da, (size_t)nblk * K * sizeof(double));
     sync(ctx);
   });

@@ -2,15 +2,38 @@
 // gp.py:242-277 (_predict_scaled) and gp.py:332-348 (_joint_factor) with
 // _covariance.py:86-108, 220-252).
 //
-// One wave per block [neighbours..., target].  The covariance lower triangle
-// is held packed in LDS (row i at i(i+1)/2), factorised right-looking with the
-// chunk jitter level, and a = L^-T e_K is formed by a column sweep.  Mode 0
-// writes a (a column of the sparse inverse Cholesky factor); mode 1 writes
-// mean = -a[0:K-1].resid[nb] / a[K-1] and var = 1 / a[K-1]^2.
+// pred_mfma_kernel (blocks of 48 to 143 points, e.g. m_pred = 140): one
+// 512-thread workgroup per block.  The covariance is held as 16 x 16 tiles of
+// its lower triangle in registers, in the matrix-core accumulator layout,
+// spread over the four waves.  The Cholesky factorisation is right-looking by
+// 16-column panels:
+//   * the panel's diagonal tile is factorised in FP64 by one wave (rows in
+//     lanes, column broadcasts by readlane);
+//   * the rows below are solved against it in FP64, one row per thread;
+//   * the trailing update A22 -= L21 L21^T runs on the matrix cores: FP64
+//     16x16x4 MFMA in FP64 mode, FP32 16x16x4 MFMA (FP32 accumulation,
+//     subtracted from the FP64 tiles) in mixed mode.
+// Mode 1 (marginal prediction) appends the residual vector r~ = [r_nb, 0] as
+// one more row: the factor of [[Sigma, r~], [r~^T, inf]] has L^-1 r~ as its
+// last row, so with a = L^-T e_K
+//   a . r~ = (L^-1 r~)_K,  mean = -(a . r~) / a_K = -(L^-1 r~)_K L_KK,
+//   var = 1 / a_K^2 = L_KK^2
+// and no back substitution is needed.  Mode 0 (columns of the sparse inverse
+// factor) keeps each factorised panel in a per-workgroup scratch area and
+// forms a = L^-T e_K by a column sweep.
+//
+// pred_blocks_kernel (smaller and larger blocks): one wave per block, the covariance lower
+// triangle packed in LDS, factorised right-looking; a = L^-T e_K by a column
+// sweep.
+//
+// The chunk jitter level (_covariance.py:241-252) adds 10^p tr(Sigma)/K to
+// the K real diagonal entries.
 
 #include "svh_internal.h"
 
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 namespace {
@@ -163,6 +186,268 @@ __global__ __launch_bounds__(kWave) void pred_blocks_kernel(PredArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ tiled
+constexpr int kPT = 512;       // threads per block of pred_mfma_kernel (8 waves)
+constexpr int kPTiles = 9;     // at most 9 x 16 = 144 rows (K + 1 <= 144)
+constexpr int kPSlots = 6;     // ceil(45 tiles / 8 waves)
+constexpr int kPStride = 17;   // panel row stride (doubles) in LDS
+
+typedef double pdvec4 __attribute__((ext_vector_type(4)));
+typedef float pfvec4 __attribute__((ext_vector_type(4)));
+
+// Row of accumulator register v held by lane group lk = lane >> 4 (the column
+// is lane & 15): FP64 16x16x4 MFMA D[lk + 4v][.], FP32 16x16x4 MFMA
+// D[4 lk + v][.].  Operands: lane L carries A[L & 15][L >> 4] and
+// B[L >> 4][L & 15] for both.
+template <bool MIX>
+__device__ __forceinline__ int acc_row(int lk, int v) {
+  return MIX ? 4 * lk + v : lk + 4 * v;
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+struct PredTileArgs {
+  double* scratch;   // mode 0: gridDim.x * Kp * Kp doubles (factor rows)
+};
+
+template <bool MIX>
+__global__ __launch_bounds__(kPT) void pred_mfma_kernel(PredArgs a, PredTileArgs ta) {
+  extern __shared__ double lds[];
+  const int Kb = a.K, d = a.d, mode = a.mode;
+  const int aug = (mode == 1) ? 1 : 0;
+  const int Kp = ((Kb + aug + 15) / 16) * 16;
+  const int KT = Kp / 16;
+  const int ntile = KT * (KT + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);   // wave-uniform
+  const int li = lane & 15, lk = lane >> 4;
+  double* xs = lds;                   // Kp * d scaled coordinates
+  double* rv = xs + Kp * d;           // Kp: residual of each neighbour (mode 1)
+  double* msk = rv + Kp;              // Kp: nugget mask
+  double* P = msk + Kp;               // Kp * kPStride: current panel
+  double* misc = P + Kp * kPStride;   // [0..15] 1/L_cc, [16] L_KK, [17] (L^-1 r~)_K, [18] fail,
+                                      // [32..63] column broadcast
+  const double kInf = __longlong_as_double(0x7ff0000000000000ll);
+  // this wave's tiles: tile t = 8 s + wid, (I, J) in row-major lower order
+  int tI[kPSlots], tJ[kPSlots];
+#pragma unroll
+  for (int s = 0; s < kPSlots; ++s) {
+    const int t = 8 * s + wid;
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+    tI[s] = (t < ntile) ? I : -1;
+    tJ[s] = (t < ntile) ? t - I * (I + 1) / 2 : -1;
+  }
+  double* scr = (mode == 0) ? ta.scratch + (int64_t)blockIdx.x * Kp * Kp : nullptr;
+  for (int64_t b = blockIdx.x; b < a.nblk; b += gridDim.x) {
+    const int ch = (int)(b / a.chunk_step);
+    const int lev = a.level[ch];
+    __syncthreads();
+    for (int e = tid; e < Kp * d; e += kPT) {
+      const int t = e / d, l = e - t * d;
+      double v = 0.0;
+      if (t < Kb) {
+        const double* src;
+        if (t < Kb - 1) {
+          src = a.Xn + a.nb[b * a.nb_stride + t] * d;
+        } else {
+          const int64_t ti = a.tg ? a.tg[b * a.tg_stride] : b;
+          src = a.Xt + ti * d;
+        }
+        v = src[l] * a.inv_lam[l];
+      }
+      xs[e] = v;
+    }
+    for (int t = tid; t < Kp; t += kPT) {
+      msk[t] = (t < Kb) ? (a.mask ? a.mask[b * Kb + t] : ((t == Kb - 1 && a.noise_free) ? 0.0 : 1.0))
+                        : 0.0;
+      rv[t] = (aug && t < Kb - 1) ? a.resid[a.nb[b * a.nb_stride + t]] : 0.0;
+    }
+    if (tid < 32) misc[tid] = 0.0;
+    __syncthreads();
+    double jit = 0.0;
+    if (lev > 0) {
+      double tr = 0.0;
+      for (int t = 0; t < Kb; ++t) tr += a.var * (1.0 + a.nug * msk[t]);
+      jit = kJitDevP[lev] * (tr / Kb);
+    }
+    // ---- covariance tiles, one 16-column panel at a time through LDS
+    // (padding rows: identity; appended row: r~ and an infinite diagonal)
+    pdvec4 T[kPSlots];
+    for (int J = 0; J < KT; ++J) {
+      const int npr = Kp - 16 * J;
+      __syncthreads();
+      for (int e = tid; e < npr * 16; e += kPT) {
+        const int row = 16 * J + (e >> 4), col = 16 * J + (e & 15);
+        double val = 0.0;
+        if (row < Kb && col < Kb) {
+          if (row == col) {
+            val = a.var * (1.0 + a.nug * msk[row]) + jit;
+          } else {
+            double s2 = 0.0;
+            for (int l = 0; l < d; ++l) {
+              const double t = xs[row * d + l] - xs[col * d + l];
+              s2 = fma(t, t, s2);
+            }
+            val = a.var * matern_M(a.nu_code, sqrt(s2));
+          }
+        } else if (aug && row == Kb) {
+          val = (col < Kb - 1) ? rv[col] : (col == Kb ? kInf : 0.0);
+        } else if (row == col) {
+          val = 1.0;
+        }
+        P[(e >> 4) * kPStride + (e & 15)] = val;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < kPSlots; ++s)
+        if (tJ[s] == J) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            T[s][v] = P[(16 * (tI[s] - J) + acc_row<MIX>(lk, v)) * kPStride + li];
+        }
+    }
+    // ---- right-looking factorisation by 16-column panels
+    for (int J = 0; J < KT; ++J) {
+      const int npr = Kp - 16 * J;   // panel rows
+      __syncthreads();               // previous panel's readers are done
+#pragma unroll
+      for (int s = 0; s < kPSlots; ++s)
+        if (tJ[s] == J) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            P[(16 * (tI[s] - J) + acc_row<MIX>(lk, v)) * kPStride + li] = T[s][v];
+        }
+      __syncthreads();
+      // diagonal tile: FP64, lane r holds row r & 15 (four copies per wave).
+      // Step c: the rows' current column-c values go through LDS (one
+      // write, broadcast reads of all 16; a wave's LDS operations complete in
+      // order), the pivot among them; buffers alternate between steps.
+      if (wid == 0) {
+        const int r = li;
+        double R[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) R[c] = (c <= r) ? P[r * kPStride + c] : 0.0;
+        double dg = 1.0;
+        bool f = false;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          double* cb = misc + 32 + 16 * (c & 1);
+          if (lane < 16) cb[r] = R[c];
+          double g[16];
+#pragma unroll
+          for (int c2 = c; c2 < 16; ++c2) g[c2] = cb[c2];
+          const double piv = g[c];
+          const double lpp = sqrt(piv);
+          const double inv = 1.0 / lpp;
+          if (16 * J + c < Kb && !(piv > 0.0)) f = true;
+          if (16 * J + c == Kb - 1) dg = lpp;   // L_KK
+          R[c] = (r > c) ? R[c] * inv : ((r == c) ? lpp : R[c]);
+#pragma unroll
+          for (int c2 = c + 1; c2 < 16; ++c2)
+            if (r >= c2) R[c2] = fma(-R[c], g[c2] * inv, R[c2]);
+        }
+        if (lane < 16) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c <= r) P[r * kPStride + c] = R[c];
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c == r) misc[r] = 1.0 / R[c];
+        }
+        if (lane == 0) {
+          if (f) misc[18] = 1.0;
+          if (Kb - 1 >= 16 * J && Kb - 1 < 16 * J + 16) misc[16] = dg;
+        }
+      }
+      __syncthreads();
+      // rows below the diagonal tile: x L_JJ^T = a, one row per thread
+      // (npr - 16 <= 128 rows: one pass)
+      if (16 + tid < npr) {
+        const int rr = 16 + tid;
+        double x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = P[rr * kPStride + c];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          x[c] *= misc[c];
+#pragma unroll
+          for (int c2 = c + 1; c2 < 16; ++c2) x[c2] = fma(-x[c], P[c2 * kPStride + c], x[c2]);
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) P[rr * kPStride + c] = x[c];
+      }
+      __syncthreads();
+      if (aug && tid == 0 && Kb - 1 >= 16 * J && Kb - 1 < 16 * J + 16)
+        misc[17] = P[(Kb - 16 * J) * kPStride + (Kb - 1 - 16 * J)];
+      if (mode == 0) {
+        // keep the factorised panel: rows 16J.., columns 16J..16J+15
+        for (int e = tid; e < npr * 16; e += kPT) {
+          const int rr = e >> 4, c = e & 15;
+          scr[(int64_t)(16 * J + rr) * Kp + 16 * J + c] = P[rr * kPStride + c];
+        }
+      }
+      // trailing update of this wave's tiles (I, K'), K' > J, on the matrix cores
+#pragma unroll
+      for (int s = 0; s < kPSlots; ++s)
+        if (tJ[s] > J) {
+          const double* pa = P + (16 * (tI[s] - J) + li) * kPStride;
+          const double* pb = P + (16 * (tJ[s] - J) + li) * kPStride;
+          if constexpr (!MIX) {
+#pragma unroll
+            for (int c4 = 0; c4 < 4; ++c4)
+              T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(-pa[4 * c4 + lk], pb[4 * c4 + lk], T[s],
+                                                          0, 0, 0);
+          } else {
+            pfvec4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c4 = 0; c4 < 4; ++c4)
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)pa[4 * c4 + lk],
+                                                         (float)pb[4 * c4 + lk], acc, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) T[s][v] -= (double)acc[v];
+          }
+        }
+    }
+    __syncthreads();
+    const bool failed = misc[18] != 0.0;
+    if (failed) {
+      if (tid == 0) atomicOr(a.fail + ch, 1);
+      continue;
+    }
+    if (mode == 1) {
+      if (tid == 0) {
+        const double LK = misc[16];
+        a.mean[b] = -misc[17] * LK;
+        a.varo[b] = LK * LK;
+      }
+    } else if (wid == 0) {
+      // a = L^-T e_K by a column sweep over the kept factor rows:
+      // a_i = t_i / L_ii, then t_j -= L_ij a_i for j < i
+      double tv[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) tv[j] = (lane + 64 * j == Kb - 1) ? 1.0 : 0.0;
+      for (int i = Kb - 1; i >= 0; --i) {
+        const double* Li = scr + (int64_t)i * Kp;
+        const int sl = i >> 6;
+        const double ti = readlane_d(sl == 0 ? tv[0] : (sl == 1 ? tv[1] : tv[2]), i & 63);
+        const double ai = ti / Li[i];
+        if (lane == 0) a.out_a[b * Kb + i] = ai;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int c = lane + 64 * j;
+          if (c < i) tv[j] = fma(-Li[c], ai, tv[j]);
+        }
+      }
+    }
+  }
+}
+
 __global__ void p_fill_int(int* p, int v, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -189,7 +474,8 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
                            const double* dXt, const int64_t* dtg, int tg_stride,
                            const double* dmask, int noise_free, int64_t nblk, int K, int d,
                            const double* theta, double nu, const double* dresid, int mode,
-                           double* dout_a, double* dmean, double* dvar, int* jitter_power_max) {
+                           double* dout_a, double* dmean, double* dvar, int* jitter_power_max,
+                           int precision) {
   SVH_REQUIRE(K >= 1, "predict: empty block");
   int nu_code;
   if (nu == 0.5)
@@ -202,7 +488,17 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
     throw SvhError{SVH_ERR_ARG, "predict: nu must be 0.5, 1.5 or 2.5"};
   if (nblk == 0) return;
   hipStream_t s = ctx->stream;
-  const size_t lds = (size_t)(K * (K + 1) / 2 + K * d + 2 * K) * sizeof(double);
+  // tiled matrix-core kernel while K (+1 appended row) fits 9 tiles of 16
+  const int Kp = ((K + (mode == 1 ? 1 : 0) + 15) / 16) * 16;
+  // SVH_PRED_KERNEL=packed selects the one-wave packed-LDS kernel (A/B timing)
+  static const bool force_packed = [] {
+    const char* e = std::getenv("SVH_PRED_KERNEL");
+    return e && std::string(e) == "packed";
+  }();
+  // small blocks (fewer than 4 tiles) run faster one wave per block
+  const bool tiled = Kp >= 64 && Kp <= 16 * kPTiles && !force_packed;
+  const size_t lds = tiled ? (size_t)(Kp * d + 2 * Kp + Kp * kPStride + 64) * sizeof(double)
+                           : (size_t)(K * (K + 1) / 2 + K * d + 2 * K) * sizeof(double);
   SVH_REQUIRE(lds <= 160 * 1024, "predict: block too large for LDS");
   double hl[32];
   for (int l = 0; l < d; ++l) hl[l] = 1.0 / std::exp(theta[1 + l]);
@@ -242,12 +538,22 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
   a.mode = mode;
   a.var = std::exp(theta[0]);
   a.nug = std::exp(theta[1 + d]);
-  const int G = (int)std::min<int64_t>(nblk, 8192);
+  const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+  const int G = (int)std::min<int64_t>(nblk, tiled ? (mode == 0 ? 2 : 4) * ncu : 8192);
+  PredTileArgs ta;
+  ta.scratch = (tiled && mode == 0)
+                   ? ctx->arena.get_t<double>("p.scratch", (size_t)G * Kp * Kp)
+                   : nullptr;
   int* hn = static_cast<int*>(ctx->stage.get(sizeof(int)));
   int rounds = 0;
   for (;;) {
     ctx->timers.begin(s, "predict_blocks");
-    hipLaunchKernelGGL(pred_blocks_kernel, dim3(G), dim3(kWave), lds, s, a);
+    if (!tiled)
+      hipLaunchKernelGGL(pred_blocks_kernel, dim3(G), dim3(kWave), lds, s, a);
+    else if (precision == SVH_MIXED)
+      hipLaunchKernelGGL(pred_mfma_kernel<true>, dim3(G), dim3(kPT), lds, s, a, ta);
+    else
+      hipLaunchKernelGGL(pred_mfma_kernel<false>, dim3(G), dim3(kPT), lds, s, a, ta);
     ctx->timers.end(s);
     hipLaunchKernelGGL(p_fill_int, dim3(1), dim3(1), 0, s, nfail, 0, (int64_t)1);
     hipLaunchKernelGGL(p_escalate, dim3((nchunk + 255) / 256), dim3(256), 0, s, level, fail,

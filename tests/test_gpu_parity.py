@@ -201,6 +201,90 @@ def test_predict_matches_oracle(gpu):
     assert _rel(va, va0) < 1e-8
 
 
+@pytest.mark.parametrize("m", [1, 15, 16, 31, 140, 142, 150])
+def test_predict_block_sizes_match_oracle(gpu, m):
+    """Block sizes around the 16-row tiles of the matrix-core kernel: the
+    appended residual row in the target's tile (m = 140, 142) or the next
+    (m = 15, 31), the 144-row limit (m = 142) and the packed kernel beyond it
+    (m = 150)."""
+    rng = np.random.default_rng(40 + m)
+    n, d, ns = 1200, 3, 64
+    X = rng.random((n, d))
+    y = np.sin(4 * X[:, 0]) + X[:, 1] * X[:, 2]
+    gpo = O.OracleGP(m_est=10, m_pred=m, n_est=n, max_iter=1, var_correction=False).fit(X, y)
+    resid = gpo.y_ - O.design(gpo.X_, gpo.trend) @ gpo.beta_
+    Xt = rng.random((ns, d))
+    mu0, va0 = gpo._predict_scaled(Xt, gpo.X_, resid, m)
+    mu, va = gpu.predict(gpo.X_, resid, Xt, m, gpo.theta_, 2.5)
+    assert _rel(mu, mu0) < 1e-9
+    assert _rel(va, va0) < 1e-8
+    # observed targets (nugget on the target too)
+    mu0, va0 = gpo._predict_scaled(Xt, gpo.X_, resid, m, noise_free=False)
+    mu, va = gpu.predict(gpo.X_, resid, Xt, m, gpo.theta_, 2.5, noise_free=False)
+    assert _rel(mu, mu0) < 1e-9
+    assert _rel(va, va0) < 1e-8
+
+
+def test_predict_mixed_precision_error_is_small(gpu):
+    """FP32 matrix-core trailing updates (mixed mode) at m = 140: the
+    predictive mean and variance stay within 1e-4 relative of FP64."""
+    rng = np.random.default_rng(33)
+    n, d, ns, m = 3000, 4, 200, 140
+    X = rng.random((n, d))
+    resid = np.sin(4 * X[:, 0]) + X[:, 1] * X[:, 2]
+    theta = np.log([1.0, 0.3, 0.4, 0.8, 2.0, 1e-3])
+    Xt = rng.random((ns, d))
+    mu, va = gpu.predict(X, resid, Xt, m, theta, 2.5)
+    mu2, va2 = gpu.predict(X, resid, Xt, m, theta, 2.5, precision="mixed")
+    assert _rel(mu2, mu) < 1e-4
+    assert _rel(va2, va) < 1e-4
+
+
+def test_predict_jitter_on_duplicate_training_points(gpu):
+    """Training points repeated exactly with a zero nugget make the prediction
+    blocks singular: the chunk jitter ladder (1e-12 ... ) applies as in the
+    oracle."""
+    rng = np.random.default_rng(34)
+    n, d, ns, m = 400, 2, 30, 140
+    X0 = rng.random((n // 2, d))
+    X = np.vstack([X0, X0])
+    resid = np.sin(3 * X[:, 0])
+    theta = np.array([0.0, math.log(0.3), math.log(0.5), math.log(1e-300)])   # nugget ~ 0
+    gpo = O.OracleGP(m_pred=m, nugget=None)
+    gpo.theta_, gpo.d_ = theta, d
+    gpo._nu_fit = gpo.nu = 2.5
+    Xt = rng.random((ns, d))
+    mu0, va0 = gpo._predict_scaled(Xt, X, resid, m)
+    mu, va = gpu.predict(X, resid, Xt, m, theta, 2.5)
+    assert _rel(mu, mu0) < 1e-5
+    assert _rel(va, va0) < 1e-5
+
+
+def test_block_columns_at_m140_match_oracle(gpu):
+    rng = np.random.default_rng(35)
+    n, d, ns, m = 600, 3, 24, 140
+    X = rng.random((n, d))
+    theta = np.log([1.3, 0.4, 0.6, 0.9, 1e-3])
+    Xall = np.vstack([X, rng.random((ns, d))])
+    Xsc = Xall / np.exp(theta[1:1 + d])
+    order = np.concatenate([O.maximin_order(Xsc[:n]), O.maximin_order(Xsc[n:]) + n])
+    nn = O.find_ordered_nn(Xsc[order], m, start=n)
+    rows = np.arange(n, n + ns)
+    blk = np.concatenate([nn[rows, 1:m + 1], rows[:, None]], axis=1)
+    mask = np.ones((ns, m + 1))
+    mask[:, m] = 0.0
+    mask[:, :m] = (blk[:, :m] < n)
+    a = gpu.block_columns(Xall[order], blk, mask, theta, 2.5)
+    Xb = Xall[order][blk]
+    Sig, _ = O.block_cov(Xb, np.exp(theta[1:1 + d]), math.exp(theta[0]), math.exp(theta[-1]),
+                         2.5, derivs=False, nug_mask=mask)
+    L, _ = O.batch_chol(Sig)
+    E = np.zeros((ns, m + 1, 1))
+    E[:, m, 0] = 1.0
+    a0 = O.backsolve_LT(L, E)[:, :, 0]
+    assert _rel(a, a0) < 1e-9
+
+
 def test_block_columns_match_oracle(gpu):
     rng = np.random.default_rng(32)
     n, d, ns, m = 500, 3, 40, 30
