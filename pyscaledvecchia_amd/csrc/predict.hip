@@ -5,10 +5,11 @@
 // pred_mfma_kernel (blocks of 48 to 143 points, e.g. m_pred = 140): one
 // 512-thread workgroup per block.  The covariance is held as 16 x 16 tiles of
 // its lower triangle in registers, in the matrix-core accumulator layout,
-// spread over the four waves.  The Cholesky factorisation is right-looking by
-// 16-column panels:
+// spread over the eight waves.  It is assembled one 16-column panel at a time
+// through LDS.  The Cholesky factorisation is right-looking by 16-column
+// panels:
 //   * the panel's diagonal tile is factorised in FP64 by one wave (rows in
-//     lanes, column broadcasts by readlane);
+//     lanes, column values broadcast through LDS);
 //   * the rows below are solved against it in FP64, one row per thread;
 //   * the trailing update A22 -= L21 L21^T runs on the matrix cores: FP64
 //     16x16x4 MFMA in FP64 mode, FP32 16x16x4 MFMA (FP32 accumulation,
@@ -22,9 +23,9 @@
 // factor) keeps each factorised panel in a per-workgroup scratch area and
 // forms a = L^-T e_K by a column sweep.
 //
-// pred_blocks_kernel (smaller and larger blocks): one wave per block, the covariance lower
-// triangle packed in LDS, factorised right-looking; a = L^-T e_K by a column
-// sweep.
+// pred_blocks_kernel (smaller and larger blocks): one wave per block, the
+// covariance lower triangle packed in LDS, factorised right-looking;
+// a = L^-T e_K by a column sweep.
 //
 // The chunk jitter level (_covariance.py:241-252) adds 10^p tr(Sigma)/K to
 // the K real diagonal entries.
