@@ -163,6 +163,44 @@ int svh_batch_set_reorder(svh_ctx* ctx, int32_t i, int32_t rebuilt);
 int svh_batch_eval(svh_ctx* ctx, const double* theta, double* ll, double* grad, double* fisher,
                    double* beta, int32_t* jitter_rounds);
 
+/* ------------------------------------------------------------ warm start --- */
+/* A small fully connected network that maps summary features of a dataset to
+ * initial log-parameters for Fisher scoring.  No counterpart in the
+ * reference, whose starting point (gp.py:169-177) stays the fallback.
+ * One network per context.  x (n_in) -> [Linear -> GELU] x n_layers (width
+ * n_hidden) -> Linear (n_out); parameters are one float32 vector, per layer
+ * W (fan_in x fan_out, row-major) then b.  Matrix products run as bf16 x bf16
+ * with float32 accumulation (bf16 weights); loss = masked mean squared error
+ * sum(M (out - T)^2) / sum(M); Adam(W) updates in float32.
+ *   init:        allocate for batches up to max_batch rows, parameters zero
+ *   set/get:     the flat parameter vector (set also resets the Adam state)
+ *   forward:     out (B x n_out) for X (B x n_in)
+ *   loss_grad:   loss and gradient of a batch, no update (checking)
+ *   load_data:   training rows X (n x n_in), targets T and masks M (n x n_out)
+ *   train_epoch: one pass in the order perm (n indices), minibatches of B,
+ *                one Adam step each (bias-corrected, step count kept);
+ *                *mean_loss = mean minibatch loss */
+int svh_mlp_init(svh_ctx* ctx, int32_t n_in, int32_t n_hidden, int32_t n_layers, int32_t n_out,
+                 int32_t max_batch);
+int svh_mlp_num_params(svh_ctx* ctx, int64_t* n);
+int svh_mlp_set_params(svh_ctx* ctx, const float* params);
+int svh_mlp_get_params(svh_ctx* ctx, float* params);
+int svh_mlp_forward(svh_ctx* ctx, const float* X, int32_t B, float* out);
+int svh_mlp_loss_grad(svh_ctx* ctx, const float* X, const float* T, const float* M, int32_t B,
+                      float* loss, float* grads);
+int svh_mlp_load_data(svh_ctx* ctx, const float* X, const float* T, const float* M, int32_t n);
+int svh_mlp_train_epoch(svh_ctx* ctx, const int32_t* perm, int32_t B, float lr, float beta1,
+                        float beta2, float eps, float weight_decay, float* mean_loss);
+
+/* Matern GP draws by random Fourier features, for the warm-start training
+ * datasets (no counterpart in the reference): for dataset i (n[i] x d[i]
+ * rows of X, already divided by the ranges, back to back; d[i] <= 16) with
+ * frequencies W (F x d[i]) and phases b (F), stored back to back,
+ *   y_j = amp[i] * sum_f cos(W_f . x_j + b_f). */
+int svh_simulate_rff(svh_ctx* ctx, int32_t ndata, const int64_t* n, const int32_t* d, int32_t F,
+                     const double* X, const double* W, const double* b, const double* amp,
+                     double* y);
+
 /* ------------------------------------------------------- instrumentation --- */
 /* No counterpart in the reference (which has no timers).  When enabled, the
  * library brackets each launch of its dominant kernels with HIP events on the

@@ -76,6 +76,23 @@ def _load():
     L.svh_batch_get.argtypes = [_vp, _dp, _dp, _dp, _i32p, _i32p, _i32p]
     L.svh_batch_set_reorder.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32]
     L.svh_batch_eval.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp, _i32p]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.svh_mlp_init.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32]
+    L.svh_mlp_num_params.argtypes = [_vp, _i64p]
+    L.svh_mlp_set_params.argtypes = [_vp, _fp]
+    L.svh_mlp_get_params.argtypes = [_vp, _fp]
+    L.svh_mlp_forward.argtypes = [_vp, _fp, ctypes.c_int32, _fp]
+    L.svh_mlp_loss_grad.argtypes = [_vp, _fp, _fp, _fp, ctypes.c_int32, _fp, _fp]
+    L.svh_mlp_load_data.argtypes = [_vp, _fp, _fp, _fp, ctypes.c_int32]
+    L.svh_mlp_train_epoch.argtypes = [_vp, _i32p, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+    L.svh_simulate_rff.argtypes = [_vp, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _dp, _dp,
+                                   _dp, _dp, _dp]
+    for name in ("svh_mlp_init", "svh_mlp_num_params", "svh_mlp_set_params", "svh_mlp_get_params",
+                 "svh_mlp_forward", "svh_mlp_loss_grad", "svh_mlp_load_data",
+                 "svh_mlp_train_epoch", "svh_simulate_rff"):
+        getattr(L, name).restype = ctypes.c_int
     for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
                  "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
@@ -90,7 +107,9 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_timing_enable",
             "svh_timing_read", "svh_timing_reset", "svh_batch_setup", "svh_batch_update",
             "svh_batch_fit_start", "svh_batch_fit_step", "svh_batch_get", "svh_batch_set_reorder",
-            "svh_batch_eval", "svh_ordering_batch")
+            "svh_batch_eval", "svh_ordering_batch", "svh_mlp_init", "svh_mlp_num_params",
+            "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
+            "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff")
 
 
 def device_count() -> int:
@@ -265,6 +284,90 @@ class Context:
 
 
 BP, BQ = 12, 11   # parameter / trend strides of the batched entry points
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+_fptr = ctypes.POINTER(ctypes.c_float)
+
+
+class MlpHandle:
+    """The context's warm-start network (svh_mlp_*): bf16-weight matrix-core
+    forward/backward and Adam on the device."""
+
+    def __init__(self, ctx: "Context", n_in, n_hidden, n_layers, n_out, max_batch=1024):
+        self.ctx = ctx
+        self.shape = (n_in, n_hidden, n_layers, n_out)
+        ctx._check(ctx._L.svh_mlp_init(ctx._h, n_in, n_hidden, n_layers, n_out, max_batch))
+        n = ctypes.c_int64()
+        ctx._check(ctx._L.svh_mlp_num_params(ctx._h, ctypes.byref(n)))
+        self.n_params = int(n.value)
+
+    def set_params(self, p):
+        p = _f32(p)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        self.ctx._check(self.ctx._L.svh_mlp_set_params(self.ctx._h, p.ctypes.data_as(_fptr)))
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        self.ctx._check(self.ctx._L.svh_mlp_get_params(self.ctx._h, p.ctypes.data_as(_fptr)))
+        return p
+
+    def forward(self, X):
+        X = _f32(np.atleast_2d(X))
+        if X.shape[1] != self.shape[0]:
+            raise ValueError("forward: wrong input width")
+        out = np.empty((X.shape[0], self.shape[3]), np.float32)
+        self.ctx._check(self.ctx._L.svh_mlp_forward(self.ctx._h, X.ctypes.data_as(_fptr),
+                                                    X.shape[0], out.ctypes.data_as(_fptr)))
+        return out
+
+    def loss_grad(self, X, T, M):
+        X, T, M = _f32(X), _f32(T), _f32(M)
+        loss = ctypes.c_float()
+        g = np.empty(self.n_params, np.float32)
+        self.ctx._check(self.ctx._L.svh_mlp_loss_grad(
+            self.ctx._h, X.ctypes.data_as(_fptr), T.ctypes.data_as(_fptr),
+            M.ctypes.data_as(_fptr), X.shape[0], ctypes.byref(loss), g.ctypes.data_as(_fptr)))
+        return float(loss.value), g
+
+    def load_data(self, X, T, M):
+        X, T, M = _f32(X), _f32(T), _f32(M)
+        self._n = X.shape[0]
+        self.ctx._check(self.ctx._L.svh_mlp_load_data(
+            self.ctx._h, X.ctypes.data_as(_fptr), T.ctypes.data_as(_fptr),
+            M.ctypes.data_as(_fptr), X.shape[0]))
+
+    def train_epoch(self, perm, batch, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        perm = np.ascontiguousarray(perm, dtype=np.int32)
+        if perm.size != self._n or perm.min() < 0 or perm.max() >= self._n:
+            raise ValueError("train_epoch: perm must index the loaded rows")
+        loss = ctypes.c_float()
+        self.ctx._check(self.ctx._L.svh_mlp_train_epoch(
+            self.ctx._h, perm.ctypes.data_as(_i32p), int(batch), float(lr), float(beta1),
+            float(beta2), float(eps), float(weight_decay), ctypes.byref(loss)))
+        return float(loss.value)
+
+
+def simulate_rff(ctx: "Context", Xs, Ws, bs, amps):
+    """Random-Fourier-feature draws y = amp * sum_f cos(W_f . x + b_f) for
+    several datasets (coordinates already divided by the ranges)."""
+    nd = len(Xs)
+    n = np.array([X.shape[0] for X in Xs], np.int64)
+    d = np.array([X.shape[1] for X in Xs], np.int32)
+    F = Ws[0].shape[0]
+    X = np.concatenate([_f64(x).ravel() for x in Xs])
+    W = np.concatenate([_f64(w).ravel() for w in Ws])
+    b = np.concatenate([_f64(v) for v in bs])
+    amp = _f64(amps)
+    y = np.empty(int(n.sum()))
+    ctx._check(ctx._L.svh_simulate_rff(ctx._h, nd, n.ctypes.data_as(_i64p),
+                                       d.ctypes.data_as(_i32p), F, _ptr(X), _ptr(W), _ptr(b),
+                                       _ptr(amp), _ptr(y)))
+    return np.split(y, np.cumsum(n)[:-1])
 
 
 def _i32(a):

@@ -128,13 +128,28 @@ class BatchedFit:
                 raise RuntimeError("batched Fisher scoring did not terminate")
         t_loop = time.perf_counter() - t0
         th, ll, beta, st, iters, nev = bh.get()
+        failed = [i for i in range(nd) if st[i] == _FAILED]
+        for i in failed:
+            if models[i].theta0 is not None or models[i].warm_start is None:
+                raise np.linalg.LinAlgError(f"dataset {i}: block Cholesky failed even with jitter")
+        if failed:
+            # learned starts that failed: refit those from the default start
+            redo = [models[i] for i in failed]
+            for g, i in zip(redo, failed):
+                g.theta0 = preps[i]["theta_default"].copy()
+            BatchedFit(redo, use_graph=self.use_graph, device=self.device).fit(
+                [Xs[i] for i in failed], [ys[i] for i in failed])
+            for g in redo:
+                g.theta0 = None
+                g.warm_start_used_ = False
         for i, (g, p) in enumerate(zip(models, preps)):
             if st[i] == _FAILED:
-                raise np.linalg.LinAlgError(f"dataset {i}: block Cholesky failed even with jitter")
+                continue
             dd, qq = p["d"], int(q[i])
             g.n_evals_ = int(nev[i])
             g.n_rebuilds_ = rebuilds[i]
             g.n_iter_ = int(iters[i])
+            g.warm_start_used_ = g.theta0 is None and g.warm_start is not None
             g._finish(th[i, :dd + 2], float(ll[i]), beta[i, :qq].copy(), p["rng"], p["n"])
         self.stats = dict(steps=steps, wall_s=time.perf_counter() - t_start,
                           build_s=t_build, loop_s=t_loop, reorder_s=t_reorder,

@@ -108,6 +108,7 @@ void svh_destroy(svh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   batch_release(ctx);
+  mlp_release(ctx);
   ctx->timers.release();
   ctx->arena.release();
   ctx->stage.release();

@@ -148,6 +148,8 @@ struct KernelTimers {
   }
 };
 
+struct MlpState;   // warm-start network (mlp.hip)
+
 struct svh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -157,6 +159,7 @@ struct svh_ctx {
   VecchiaState vs;
   KernelTimers timers;
   int mm_rounds = 0;  // rounds of the last maximin call
+  MlpState* mlp = nullptr;
   std::string last_error;
 };
 
@@ -173,5 +176,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
                          int need_grad, double var_penalty, double log_var_target,
                          double* ll, double* grad, double* fisher, double* beta,
                          int* jitter_power_max);
+// mlp.hip
+void mlp_release(svh_ctx* ctx);
 // Jitter factors 10^p, p = -12..-4 (index 1..9; index 0 = no jitter).
 extern const double kJitterPow10[10];

@@ -8,7 +8,9 @@ every likelihood evaluation, the prediction blocks and their kNN query --
 runs on the GPU through libsvh.so.
 
 Additive knobs (defaults reproduce the reference): `precision` ("fp64" or
-"mixed"), `theta0` (a warm start for Fisher scoring), `device`.
+"mixed"), `theta0` (a warm start for Fisher scoring), `warm_start` (a learned
+starting point: a WarmStart network, or "learned" for the packaged one),
+`device`.
 """
 
 from __future__ import annotations
@@ -46,6 +48,10 @@ class ScaledVecchiaGP:
                      search (log-likelihood, Cholesky, GLS stay FP64), then
                      FP64 refinement steps
     theta0         : optional initial log-parameters for Fisher scoring
+    warm_start     : None (default start), "learned" (the packaged network) or
+                     a WarmStart instance: initial log-parameters predicted
+                     from summary features of the data; a start whose
+                     Cholesky fails falls back to the default start
     """
 
     m_est: int = 30
@@ -63,6 +69,7 @@ class ScaledVecchiaGP:
     verbose: bool = False
     precision: str = "fp64"
     theta0: np.ndarray | None = field(default=None, repr=False)
+    warm_start: object = field(default=None, repr=False)
     device: int = 0
 
     X_: np.ndarray = field(default=None, init=False, repr=False)
@@ -151,20 +158,33 @@ class ScaledVecchiaGP:
         m = min(self.m_est, len(sub) - 1)
 
         fixed_nug = self.nugget is not None
+        default = np.empty(d + 2)
+        default[0] = math.log(max(ye.var(), 1e-8))
+        default[1:1 + d] = math.log(0.2 * math.sqrt(d))
+        default[-1] = math.log(self.nugget if fixed_nug else 0.1)
+        default = self._clip_theta(default, d)
         if self.theta0 is not None:
             theta = np.array(self.theta0, dtype=float).copy()
             if theta.shape != (d + 2,):
                 raise ValueError("theta0 must have d + 2 entries")
-            if fixed_nug:
-                theta[-1] = math.log(self.nugget)
+        elif self.warm_start is not None:
+            theta = self._warm_start_net().theta0(Xe, ye, self.nu)
         else:
-            theta = np.empty(d + 2)
-            theta[0] = math.log(max(ye.var(), 1e-8))
-            theta[1:1 + d] = math.log(0.2 * math.sqrt(d))
-            theta[-1] = math.log(self.nugget if fixed_nug else 0.1)
+            theta = default.copy()
+        if fixed_nug:
+            theta[-1] = math.log(self.nugget)
         theta = self._clip_theta(theta, d)
         return dict(n=n, d=d, rng=rng, Xe=Xe, ye=ye, Ze=Ze, m=m, theta=theta,
-                    fixed_nug=fixed_nug, lvt=math.log(max(ye.var(), 1e-8)))
+                    theta_default=default, fixed_nug=fixed_nug,
+                    lvt=math.log(max(ye.var(), 1e-8)))
+
+    def _warm_start_net(self):
+        from .warmstart import WarmStart
+        if isinstance(self.warm_start, str):
+            if self.warm_start != "learned":
+                raise ValueError("warm_start must be None, 'learned' or a WarmStart")
+            self.warm_start = WarmStart.load(device=self.device)
+        return self.warm_start
 
     def _order_structure(self, Xe, theta, m):
         """Maximin order and ordered neighbours in the scaled space
@@ -239,9 +259,24 @@ class ScaledVecchiaGP:
             return True
 
         self.history_ = []
-        theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
-                                          tol=self.tol, verbose=self.verbose,
-                                          reorder_hook=reorder_hook, history=self.history_)
+        self.warm_start_used_ = self.theta0 is None and self.warm_start is not None
+        try:
+            theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
+                                              tol=self.tol, verbose=self.verbose,
+                                              reorder_hook=reorder_hook, history=self.history_)
+        except np.linalg.LinAlgError:
+            if not self.warm_start_used_:
+                raise
+            # the learned start failed: the default start is the fallback
+            self.warm_start_used_ = False
+            theta = prep["theta_default"].copy()
+            rebuild(theta)
+            last_ranges = np.exp(theta[1:1 + d]).copy()
+            self.history_ = []
+            theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
+                                              tol=self.tol, verbose=self.verbose,
+                                              reorder_hook=reorder_hook, history=self.history_)
+        self.n_iter_ = len(self.history_)
         self.n_refine_ = 0
         if self.precision == "mixed":
             # Refinement: the mixed search used FP64 log-likelihoods but FP32
