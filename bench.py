@@ -68,7 +68,6 @@ def c3_datasets(nd=64, seed=1003):
             F = 2048
             # Matern spectral density: omega = z * sqrt(2 nu / chi2_{2nu}) (multivariate t)
             w = rng.standard_normal((F, d)) * np.sqrt(2 * nu / rng.chisquare(2 * nu, (F, 1)))
-            w *= math.sqrt(2 * nu)
             b = rng.uniform(0, 2 * math.pi, F)
             y = math.sqrt(2.0 / F) * np.cos(Xs @ w.T + b).sum(1)
             y += math.sqrt(tau) * rng.standard_normal(n)
@@ -108,6 +107,97 @@ def run_c3(args):
         out["sequential_s"] = t_seq
         out["speedup_vs_sequential"] = t_seq / t_batch
         out["max_rel_loglik_gap_vs_sequential"] = max(gaps)
+    print(json.dumps(out))
+
+
+def run_c4(args):
+    """Learned warm start on held-out datasets from the prior at n = 50,000:
+    each fitted twice by the batched fitter, from the default start and from
+    the network's start."""
+    from pyscaledvecchia_amd import warmstart as W
+    ws = W.WarmStart.load()
+    t0 = time.perf_counter()
+    rec = W.evaluate(ws, count=args.ndata, n=args.n_c4, batch=64, verbose=False)
+    s = W.summarize(rec)
+    out = {"metric": "warm start: Fisher-scoring iterations saved (mean), held-out datasets",
+           "value": s["iters_saved_mean"], "unit": "iterations", "n_gpus": 1,
+           "higher_is_better": True, "data": "synthetic (prior draws by random Fourier features)",
+           "dtype": "f64 fit, bf16-weight network", "vs_baseline": None,
+           "config": {"workload": f"C4 warm start, {args.ndata} held-out datasets, n={args.n_c4}",
+                      "datasets": args.ndata, "n": args.n_c4},
+           "summary": s, "seconds": time.perf_counter() - t0}
+    print(json.dumps(out))
+
+
+PEAK_FP64_MFMA_TFLOPS = 78.6
+
+
+def c5_inputs(n, ns, d=10, seed=1005):
+    """Prediction workload after the large-prediction configuration: dims 1-3
+    from a two-component Gaussian mixture (means 0.3 / 0.7, sd 0.1, clipped
+    to [0, 1]), the rest uniform; y from the sin-sum test function extended
+    to 10 inputs."""
+    rng = np.random.default_rng(seed)
+
+    def draw(k):
+        X = rng.random((k, d))
+        comp = rng.random(k) < 0.5
+        mu = np.where(comp[:, None], 0.3, 0.7)
+        X[:, :3] = np.clip(mu + 0.1 * rng.standard_normal((k, 3)), 0.0, 1.0)
+        return X
+    a = np.array([1.0, 0.8, 0.6, 0.4, 0.2, 0.05, 0.0, 0.0, 0.0, 0.0])[:d]
+    ell = np.array([0.5, 0.7, 1.0, 1.5, 2.0, 3.0, 1.0, 1.0, 1.0, 1.0])[:d]
+    X, Xs = draw(n), draw(ns)
+    y = (a * np.sin(2 * np.pi * X / ell)).sum(1) + 0.1 * X[:, 0] * X[:, 1]
+    return X, y + rng.normal(0.0, 0.01, n), Xs
+
+
+def pred_block_flops(K, d):
+    """Algorithmic flops of one prediction block (FMA = 2): pair distances and
+    Matern, Cholesky K^3/3, the appended residual row, mean/variance."""
+    T = K * (K + 1) / 2
+    return T * (3 * d + 6) + K ** 3 / 3 + K ** 2 + 2 * K
+
+
+def run_c5p(args):
+    """Marginal prediction at m_pred = 140 (K = 141 blocks): kNN query plus the
+    matrix-core block kernel; inputs resident before the timed region."""
+    from pyscaledvecchia_amd import _lib
+    ctx = _lib.context()
+    d, m = 10, 140
+    X, y, Xs = c5_inputs(args.n_train, args.n_test, d)
+    theta = np.concatenate([[0.0], np.full(d, math.log(0.5)), [math.log(1e-4)]])
+    resid = (y - y.mean()) / y.std()
+    ctx.predict(X, resid, Xs[:1000], m, theta, 2.5)           # warm-up
+    res = {}
+    for prec in ("fp64", "mixed"):
+        ctx.timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            mu, va = ctx.predict(X, resid, Xs, m, theta, 2.5, precision=prec)
+        t = (time.perf_counter() - t0) / args.steps
+        ctx.timing(False)
+        res[prec] = dict(wall_s=t, blocks_ms=ctx.timing_read("predict_blocks")[0] / args.steps,
+                         knn_ms=ctx.timing_read("knn")[0] / args.steps, mu=mu, va=va)
+    flops = args.n_test * pred_block_flops(m + 1, d)
+    b64 = res["fp64"]["blocks_ms"] * 1e-3
+    achieved = flops / b64 / 1e12
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))   # noqa: E731
+    out = {"metric": "prediction blocks/s at m_pred=140 (K=141), d=10",
+           "value": args.n_test / b64, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps,
+           "higher_is_better": True, "dtype": "f64", "vs_baseline": None,
+           "data": "synthetic (GMM design, sin-sum function)",
+           "config": {"workload": f"C5-shaped marginal prediction n_train={args.n_train} "
+                                  f"n_test={args.n_test} d={d} m_pred={m}"},
+           "roofline": {"bound": "mfma", "kernel": "pred_mfma_kernel<false>",
+                        "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64_MFMA_TFLOPS,
+                        "flops_per_launch": flops, "kernel_avg_ms": 1e3 * b64},
+           "knn_ms": res["fp64"]["knn_ms"], "predict_wall_s": res["fp64"]["wall_s"],
+           "mixed": {"blocks_ms": res["mixed"]["blocks_ms"],
+                     "mean_rel_err_vs_fp64": rel(res["mixed"]["mu"], res["fp64"]["mu"]),
+                     "var_rel_err_vs_fp64": rel(res["mixed"]["va"], res["fp64"]["va"])}}
     print(json.dumps(out))
 
 
@@ -158,7 +248,11 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5p"])
+    ap.add_argument("--n-train", type=int, default=50_000)
+    ap.add_argument("--n-test", type=int, default=20_000)
+    ap.add_argument("--n-c4", type=int, default=50_000)
+    ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--ndata", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -169,6 +263,16 @@ def main():
         return
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "c5p":
+        if args.steps == 50:
+            args.steps = 5
+        run_c5p(args)
+        return
+    if args.workload == "c4":
+        if args.ndata == 64:
+            args.ndata = 256
+        run_c4(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -293,6 +397,32 @@ def main():
             "loglik": abs(gm.loglik_ - g6.loglik_) / abs(g6.loglik_),
             "theta": rel(gm.theta_, g6.theta_), "pred_mean": rel(mum, mu6),
             "pred_sd": rel(sdm, sd6), "test_points": len(Xt)}
+        # fast path: learned warm start + batched fitter (device Fisher-scoring
+        # state machine replayed as a hipGraph), FP64 kernels
+        from pyscaledvecchia_amd.warmstart import DEFAULT_NET
+        if not args.no_fast and os.path.exists(DEFAULT_NET):
+            from pyscaledvecchia_amd.batch import BatchedFit
+            ctx.timing(True)
+            ctx.timing_reset()
+            t0 = time.perf_counter()
+            gf = ScaledVecchiaGP(m_est=m, m_pred=140, nu=2.5, trend="linear", n_est=n,
+                                 warm_start="learned")
+            BatchedFit([gf]).fit([X], [y])
+            t_fast = time.perf_counter() - t0
+            ctx.timing(False)
+            muf, sdf = gf.predict(Xt, return_std=True)
+            out["fit_s_fast"] = t_fast
+            out["fast_fit"] = {
+                "evals": gf.n_evals_, "rebuilds": gf.n_rebuilds_, "iters": gf.n_iter_,
+                "iters_plain": g6.n_iter_, "warm_start_used": gf.warm_start_used_,
+                "loglik": gf.loglik_, "speedup_vs_plain_fp64": out["fit_s"] / t_fast
+                if args.precision == "fp64" else out["fit_s_fp64"] / t_fast,
+                "maximin_s": ctx.timing_read("maximin")[0] / 1e3,
+                "knn_s": ctx.timing_read("knn")[0] / 1e3}
+            out["fast_fit_vs_fp64_rel_err"] = {
+                "loglik": abs(gf.loglik_ - g6.loglik_) / abs(g6.loglik_),
+                "theta": rel(gf.theta_, g6.theta_), "pred_mean": rel(muf, mu6),
+                "pred_sd": rel(sdf, sd6)}
     # CPU baseline: the C restatement of the oracle (OpenMP over blocks, as the
     # reference's prange), full C2 structure, repeated for ~cpu-seconds
     if not args.no_cpu:

@@ -61,11 +61,27 @@ class BatchedFit:
             raise ValueError("one (X, y) per model")
         preps = []
         t0 = time.perf_counter()
-        for g, X, y in zip(models, Xs, ys):
+        ws_idx = [i for i, g in enumerate(models) if g.theta0 is None and g.warm_start is not None]
+        for i, (g, X, y) in enumerate(zip(models, Xs, ys)):
+            if i in ws_idx:
+                g._ws_theta0 = np.zeros(np.atleast_2d(X).shape[1] + 2)   # placeholder
             p = g._prepare(X, y)
             if p["d"] > 10 or g._design(np.zeros((1, p["d"]))).shape[1] > 11 or p["m"] > 31:
                 raise ValueError("batched fits need d <= 10 and m_est <= 31")
             preps.append(p)
+        if ws_idx:
+            # learned starts of all warm-started datasets in one network call
+            by_net = {}
+            for i in ws_idx:
+                by_net.setdefault(id(models[i]._warm_start_net()), []).append(i)
+            for ids in by_net.values():
+                net = models[ids[0]]._warm_start_net()
+                ths = net.theta0_many([(preps[i]["Xe"], preps[i]["ye"], models[i].nu)
+                                       for i in ids])
+                for i, th in zip(ids, ths):
+                    if models[i].nugget is not None:
+                        th[-1] = math.log(models[i].nugget)
+                    preps[i]["theta"] = models[i]._clip_theta(th, preps[i]["d"])
         mmax = max(p["m"] for p in preps)
         structs = self._structures(ctx, models, preps, [p["theta"] for p in preps], mmax)
         t_build = time.perf_counter() - t0
@@ -90,6 +106,30 @@ class BatchedFit:
         theta0 = np.zeros((nd, _lib.BP))
         for i, p in enumerate(preps):
             theta0[i, :p["d"] + 2] = p["theta"]
+        use_ws = np.zeros(nd, bool)
+        use_ws[ws_idx] = True
+        if ws_idx:
+            # start guard: keep a learned start only where its log-likelihood
+            # beats the default start's (one batched evaluation each)
+            thd = theta0.copy()
+            for i in ws_idx:
+                thd[i, :preps[i]["d"] + 2] = preps[i]["theta_default"]
+            try:
+                ll_ws = bh.eval(theta0)[0]
+                ll_def = bh.eval(thd)[0]
+                back = [i for i in ws_idx if not ll_ws[i] > ll_def[i]]
+            except np.linalg.LinAlgError:
+                back = list(ws_idx)
+            if back:
+                for i in back:
+                    preps[i]["theta"] = preps[i]["theta_default"].copy()
+                    theta0[i, :preps[i]["d"] + 2] = preps[i]["theta"]
+                    use_ws[i] = False
+                new = self._structures(ctx, [models[i] for i in back], [preps[i] for i in back],
+                                       [preps[i]["theta"] for i in back], mmax)
+                for i, st_i in zip(back, new):
+                    structs[i] = st_i
+                    bh.update(i, st_i["Xo"], st_i["yo"], st_i["Zo"], st_i["nbr"], st_i["kcnt"])
         last = [np.exp(p["theta"][1:1 + p["d"]]).copy() for p in preps]
         bh.fit_start(theta0, models[0].max_iter, models[0].tol, models[0].var_penalty,
                      self.use_graph)
@@ -149,7 +189,7 @@ class BatchedFit:
             g.n_evals_ = int(nev[i])
             g.n_rebuilds_ = rebuilds[i]
             g.n_iter_ = int(iters[i])
-            g.warm_start_used_ = g.theta0 is None and g.warm_start is not None
+            g.warm_start_used_ = bool(use_ws[i])
             g._finish(th[i, :dd + 2], float(ll[i]), beta[i, :qq].copy(), p["rng"], p["n"])
         self.stats = dict(steps=steps, wall_s=time.perf_counter() - t_start,
                           build_s=t_build, loop_s=t_loop, reorder_s=t_reorder,

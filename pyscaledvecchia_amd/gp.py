@@ -46,7 +46,7 @@ class ScaledVecchiaGP:
     precision      : 'fp64' (default) or 'mixed': FP32 matrix-core products for
                      the derivative terms and Fisher information during the
                      search (log-likelihood, Cholesky, GLS stay FP64), then
-                     FP64 refinement steps
+                     FP64 refinement steps; predictions are FP64 either way
     theta0         : optional initial log-parameters for Fisher scoring
     warm_start     : None (default start), "learned" (the packaged network) or
                      a WarmStart instance: initial log-parameters predicted
@@ -168,7 +168,10 @@ class ScaledVecchiaGP:
             if theta.shape != (d + 2,):
                 raise ValueError("theta0 must have d + 2 entries")
         elif self.warm_start is not None:
-            theta = self._warm_start_net().theta0(Xe, ye, self.nu)
+            theta = getattr(self, "_ws_theta0", None)
+            if theta is None:
+                theta = self._warm_start_net().theta0(Xe, ye, self.nu)
+            self._ws_theta0 = None
         else:
             theta = default.copy()
         if fixed_nug:
@@ -232,6 +235,7 @@ class ScaledVecchiaGP:
         rebuild(theta)
         last_ranges = np.exp(theta[1:1 + d]).copy()
         log_var_target = prep["lvt"]
+        use_ws = self.theta0 is None and self.warm_start is not None
 
         def objective(th, precision=self.precision):
             th = self._clip_theta(np.array(th, dtype=float), d)
@@ -259,7 +263,24 @@ class ScaledVecchiaGP:
             return True
 
         self.history_ = []
-        self.warm_start_used_ = self.theta0 is None and self.warm_start is not None
+        if use_ws:
+            # start guard: the learned start is kept only if its log-likelihood
+            # beats the default start's; otherwise the reference start is used
+            th_def = prep["theta_default"]
+            try:
+                ll_ws = objective(theta)[0]
+            except np.linalg.LinAlgError:
+                ll_ws = -np.inf
+            try:
+                ll_def = objective(th_def)[0]
+            except np.linalg.LinAlgError:
+                ll_def = -np.inf
+            if not ll_ws > ll_def:
+                use_ws = False
+                theta = th_def.copy()
+                rebuild(theta)
+                last_ranges = np.exp(theta[1:1 + d]).copy()
+        self.warm_start_used_ = use_ws
         try:
             theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
                                               tol=self.tol, verbose=self.verbose,
@@ -297,8 +318,11 @@ class ScaledVecchiaGP:
         """Marginal Vecchia predictions from the m nearest training points
         (gp.py:242-277): kNN query and K = m+1 blocks on the GPU."""
         m = min(m, X_tr.shape[0])
+        # FP64 blocks whatever the fit's precision: with tiny nuggets the
+        # K = m + 1 blocks are ill-conditioned and FP32 trailing updates move
+        # the predictive variance (and the b correction) visibly
         return self._ctx().predict(X_tr, y_tr_resid, Xs_std, m, self.theta_, self.nu_,
-                                   noise_free=noise_free, precision=self.precision)
+                                   noise_free=noise_free, precision="fp64")
 
     def predict(self, Xstar, return_std=False, return_var=False, m=None):
         self._check_fitted()
@@ -341,7 +365,7 @@ class ScaledVecchiaGP:
         mask[:, K - 1] = 0.0
         mask[:, :K - 1] = (block[:, :K - 1] < n).astype(float)
         av = self._ctx().block_columns(Xo, block, mask, self.theta_, self.nu_,
-                                       precision=self.precision)
+                                       precision="fp64")
         Ufull = sparse.coo_matrix((av.ravel(), (block.ravel(), np.repeat(np.arange(ns), K))),
                                   shape=(n + ns, ns)).tocsc()
         return Ufull[:n, :], Ufull[n:, :], order[n:] - n, order_obs
