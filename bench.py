@@ -253,6 +253,8 @@ def main():
     ap.add_argument("--n-test", type=int, default=20_000)
     ap.add_argument("--n-c4", type=int, default=50_000)
     ap.add_argument("--no-fast", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="built-in launch schedules (default: tune during the warm-up)")
     ap.add_argument("--ndata", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -293,6 +295,8 @@ def main():
                             [math.log(0.1)]])
     lvt = math.log(ye.var())
     ctx = _lib.context()
+    if not args.no_autotune:
+        ctx.autotune(2)       # schedules tuned by the first (warm-up) call of each shape
 
     # rebuild (timed separately, not part of the step)
     Xs = Xe / np.exp(theta[1:1 + d])
@@ -366,6 +370,7 @@ def main():
                 "kernel_avg_ms": k_ms_o / max(k_cnt_o, 1)},
         "mixed_vs_fp64_rel_err": {"loglik": abs(llmx - ll64) / abs(ll64),
                                   "grad": rel(gmx, g64), "fisher": rel(Fmx, F64)},
+        "autotune": {k: {"candidate": v[0], "ms": v[1]} for k, v in ctx.autotune_choices().items()},
         "rebuild_s": {"maximin": t_mm, "ordered_knn": t_knn},
         "maximin_detail": mm_detail,
     }

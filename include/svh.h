@@ -201,6 +201,23 @@ int svh_simulate_rff(svh_ctx* ctx, int32_t ndata, const int64_t* n, const int32_
                      const double* X, const double* W, const double* b, const double* amp,
                      double* y);
 
+/* -------------------------------------------------------------- autotune --- */
+/* Launch-schedule autotuner (no counterpart in the reference).  Tunable launch
+ * sites -- the likelihood pass-1 kernel (split-row kernel at 2..32 workgroups
+ * per CU, or the general LDS kernel) and the prediction blocks (packed kernel,
+ * or the matrix-core kernel at 1, 2, 4 workgroups per CU) -- key their shape
+ * bucket (block size, d, q, precision, log2 of the block count).
+ *   mode 0: built-in schedules (default); 1: cached choices only;
+ *   2: the first call of a new bucket times every candidate (one warm-up,
+ *      median of three, HIP events) and keeps the fastest.
+ * cache_path (may be NULL): text file "key index ms" per line, read at first
+ * use and rewritten when a bucket is tuned.  Results do not depend on the
+ * schedule except for the grouping of the likelihood's partial sums.
+ * svh_autotune_read writes the choices so far ("key index ms" lines) into
+ * buf; SVH_ERR_ARG if they do not fit in len bytes. */
+int svh_autotune(svh_ctx* ctx, int32_t mode, const char* cache_path);
+int svh_autotune_read(svh_ctx* ctx, char* buf, int32_t len);
+
 /* ------------------------------------------------------- instrumentation --- */
 /* No counterpart in the reference (which has no timers).  When enabled, the
  * library brackets each launch of its dominant kernels with HIP events on the

@@ -89,7 +89,9 @@ def _load():
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
     L.svh_simulate_rff.argtypes = [_vp, ctypes.c_int32, _i64p, _i32p, ctypes.c_int32, _dp, _dp,
                                    _dp, _dp, _dp]
-    for name in ("svh_mlp_init", "svh_mlp_num_params", "svh_mlp_set_params", "svh_mlp_get_params",
+    L.svh_autotune.argtypes = [_vp, ctypes.c_int32, ctypes.c_char_p]
+    L.svh_autotune_read.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int32]
+    for name in ("svh_autotune", "svh_autotune_read", "svh_mlp_init", "svh_mlp_num_params", "svh_mlp_set_params", "svh_mlp_get_params",
                  "svh_mlp_forward", "svh_mlp_loss_grad", "svh_mlp_load_data",
                  "svh_mlp_train_epoch", "svh_simulate_rff"):
         getattr(L, name).restype = ctypes.c_int
@@ -109,7 +111,8 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_batch_fit_start", "svh_batch_fit_step", "svh_batch_get", "svh_batch_set_reorder",
             "svh_batch_eval", "svh_ordering_batch", "svh_mlp_init", "svh_mlp_num_params",
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
-            "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff")
+            "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
+            "svh_autotune_read")
 
 
 def device_count() -> int:
@@ -167,6 +170,24 @@ class Context:
         raise RuntimeError(msg)
 
     # ---------------------------------------------------------------- timers
+    def autotune(self, mode=2, cache_path=None):
+        """Launch-schedule autotuner: 0 = built-in schedules, 1 = cached
+        choices only, 2 = also time the candidates of new shape buckets.
+        `cache_path`: text file of choices (read at first use, rewritten when
+        a new bucket is tuned)."""
+        path = None if cache_path is None else os.fsencode(cache_path)
+        self._check(self._L.svh_autotune(self._h, int(mode), path))
+
+    def autotune_choices(self):
+        """{shape-bucket key: (candidate index, median ms)} tuned so far."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(self._L.svh_autotune_read(self._h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            k, i, ms = line.split()
+            out[k] = (int(i), float(ms))
+        return out
+
     def timing(self, enable=True):
         self._check(self._L.svh_timing_enable(self._h, int(bool(enable))))
 

@@ -1488,8 +1488,28 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   const bool use_split = !(kenv && std::string(kenv) == "lds") && vs.mmax + 1 <= kSK &&
                          d <= kSD && q <= kSQ;
   const size_t lds_split = (size_t)split_lds_doubles(d) * sizeof(double);
-  const int G = use_split ? (int)std::min<int64_t>(vs.nblk, 8192)
-                          : (int)std::min<int64_t>(vs.nblk, 2048);
+  // launch-schedule candidates for the autotuner: split kernel at 2..32
+  // workgroups per CU, then the general LDS kernel; defaults as fixed before
+  const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+  struct Cand {
+    bool split;
+    int G;
+  };
+  std::vector<Cand> cands;
+  int dflt = 0;
+  if (use_split) {
+    for (int f : {2, 4, 8, 16, 32}) cands.push_back({true, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
+    dflt = (int)cands.size();
+    cands.push_back({true, (int)std::min<int64_t>(vs.nblk, 8192)});
+    if (!(vs.mmax + 1 > kMaxKP)) cands.push_back({false, (int)std::min<int64_t>(vs.nblk, 2048)});
+  } else {
+    for (int f : {2, 4, 16}) cands.push_back({false, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
+    dflt = (int)cands.size();
+    cands.push_back({false, (int)std::min<int64_t>(vs.nblk, 2048)});
+  }
+  int Gmax = 1;
+  for (const Cand& c : cands) Gmax = std::max(Gmax, c.G);
+  int G = cands[dflt].G;
   const int G2 = (int)std::min<int64_t>((vs.nblk + 255) / 256, 512);
   const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC) * sizeof(double) +
                      (size_t)KP * sizeof(int);
@@ -1507,7 +1527,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.fail = fail;
   a.inv_lam = inv_lam;
   a.cache = ctx->arena.get_t<double>("v.cache", (size_t)vs.nblk * NC);
-  a.partial = ctx->arena.get_t<double>("v.partial", (size_t)G * NACC + 1);
+  a.partial = ctx->arena.get_t<double>("v.partial", (size_t)Gmax * NACC + 1);
   a.nblk = vs.nblk;
   a.d = d;
   a.q = q;
@@ -1531,22 +1551,52 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   double* dout = ctx->arena.get_t<double>("v.out", NOUT);
   double* hout = static_cast<double*>(ctx->stage.get(NOUT * sizeof(double)));
 
+  struct Launch {
+    hipStream_t s;
+    const std::vector<Cand>* cands;
+    int nu_code, KP;
+    size_t lds_split, lds;
+    const LLArgs* a;
+    bool mixed;
+    static void run(void* u, int i) {
+      const Launch& L = *static_cast<Launch*>(u);
+      const Cand& c = (*L.cands)[i];
+      if (c.split) {
+        switch (L.nu_code) {
+          case 0: launch_split_nu<0>(L.s, c.G, L.lds_split, *L.a, L.mixed); break;
+          case 1: launch_split_nu<1>(L.s, c.G, L.lds_split, *L.a, L.mixed); break;
+          default: launch_split_nu<2>(L.s, c.G, L.lds_split, *L.a, L.mixed); break;
+        }
+      } else {
+        switch (L.KP) {
+          case 32: launch_pass1<32>(L.s, c.G, L.lds, *L.a); break;
+          case 64: launch_pass1<64>(L.s, c.G, L.lds, *L.a); break;
+          default: launch_pass1<128>(L.s, c.G, L.lds, *L.a); break;
+        }
+      }
+    }
+  };
+  Launch lau{s, &cands, nu_code, KP, lds_split, lds, &a, precision == SVH_MIXED};
+  if (ctx->tune.mode != 0 && !a.prof) {
+    char key[160];
+    int lg = 0;
+    while ((int64_t(1) << (lg + 1)) <= vs.nblk) ++lg;
+    std::snprintf(key, sizeof key, "ll_pass1:K%d:d%d:q%d:g%d:nu%d:p%d:nb2^%d", vs.mmax + 1, d, q,
+                  need_grad, nu_code, precision, lg);
+    const int pick = tune_choose(ctx, key, (int)cands.size(), dflt, &Launch::run, &lau);
+    G = cands[pick].G;
+    std::vector<Cand> one{cands[pick]};
+    cands.swap(one);
+    int_fill(ctx, fail, 0, nchunk);   // tuning launches may have flagged chunks
+  } else {
+    std::vector<Cand> one{cands[dflt]};
+    cands.swap(one);
+  }
+
   int rounds = 0;
   for (;;) {
     ctx->timers.begin(s, "vecchia_pass1");
-    if (use_split) {
-      switch (nu_code) {
-        case 0: launch_split_nu<0>(s, G, lds_split, a, precision == SVH_MIXED); break;
-        case 1: launch_split_nu<1>(s, G, lds_split, a, precision == SVH_MIXED); break;
-        default: launch_split_nu<2>(s, G, lds_split, a, precision == SVH_MIXED); break;
-      }
-    } else {
-      switch (KP) {
-        case 32: launch_pass1<32>(s, G, lds, a); break;
-        case 64: launch_pass1<64>(s, G, lds, a); break;
-        default: launch_pass1<128>(s, G, lds, a); break;
-      }
-    }
+    Launch::run(&lau, 0);
     SVH_HIP(hipGetLastError());
     ctx->timers.end(s);
     if (NACC > 0)

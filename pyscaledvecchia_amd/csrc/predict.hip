@@ -497,10 +497,12 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
     return e && std::string(e) == "packed";
   }();
   // small blocks (fewer than 4 tiles) run faster one wave per block
-  const bool tiled = Kp >= 64 && Kp <= 16 * kPTiles && !force_packed;
-  const size_t lds = tiled ? (size_t)(Kp * d + 2 * Kp + Kp * kPStride + 64) * sizeof(double)
-                           : (size_t)(K * (K + 1) / 2 + K * d + 2 * K) * sizeof(double);
-  SVH_REQUIRE(lds <= 160 * 1024, "predict: block too large for LDS");
+  const bool tiled_ok = Kp >= 32 && Kp <= 16 * kPTiles && !force_packed;
+  const bool tiled_dflt = tiled_ok && Kp >= 64;
+  const size_t lds_t = (size_t)(Kp * d + 2 * Kp + Kp * kPStride + 64) * sizeof(double);
+  const size_t lds_p = (size_t)(K * (K + 1) / 2 + K * d + 2 * K) * sizeof(double);
+  SVH_REQUIRE((tiled_ok ? lds_t : lds_p) <= 160 * 1024, "predict: block too large for LDS");
+  const bool packed_ok = lds_p <= 160 * 1024;
   double hl[32];
   for (int l = 0; l < d; ++l) hl[l] = 1.0 / std::exp(theta[1 + l]);
   double* inv_lam = ctx->arena.get_t<double>("p.invlam", 32);
@@ -539,22 +541,65 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
   a.mode = mode;
   a.var = std::exp(theta[0]);
   a.nug = std::exp(theta[1 + d]);
+  // launch-schedule candidates for the autotuner: the packed kernel, the tiled
+  // kernel at 1, 2, 4 workgroups per CU; defaults as fixed before
   const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
-  const int G = (int)std::min<int64_t>(nblk, tiled ? (mode == 0 ? 2 : 4) * ncu : 8192);
+  struct PCand {
+    bool tiled;
+    int G;
+  };
+  std::vector<PCand> cands;
+  if (packed_ok) cands.push_back({false, (int)std::min<int64_t>(nblk, 8192)});
+  int dflt = 0;
+  if (tiled_ok) {
+    for (int f : {1, 2, 4}) {
+      if (tiled_dflt && f == (mode == 0 ? 2 : 4)) dflt = (int)cands.size();
+      cands.push_back({true, (int)std::min<int64_t>(nblk, (int64_t)f * ncu)});
+    }
+  }
+  int Gs = 1;
+  for (const PCand& c : cands)
+    if (c.tiled && mode == 0) Gs = std::max(Gs, c.G);
   PredTileArgs ta;
-  ta.scratch = (tiled && mode == 0)
-                   ? ctx->arena.get_t<double>("p.scratch", (size_t)G * Kp * Kp)
+  ta.scratch = (tiled_ok && mode == 0)
+                   ? ctx->arena.get_t<double>("p.scratch", (size_t)Gs * Kp * Kp)
                    : nullptr;
+  struct PLaunch {
+    hipStream_t s;
+    const std::vector<PCand>* cands;
+    size_t lds_t, lds_p;
+    const PredArgs* a;
+    PredTileArgs ta;
+    bool mixed;
+    static void run(void* u, int i) {
+      const PLaunch& L = *static_cast<PLaunch*>(u);
+      const PCand& c = (*L.cands)[i];
+      if (!c.tiled)
+        hipLaunchKernelGGL(pred_blocks_kernel, dim3(c.G), dim3(kWave), L.lds_p, L.s, *L.a);
+      else if (L.mixed)
+        hipLaunchKernelGGL(pred_mfma_kernel<true>, dim3(c.G), dim3(kPT), L.lds_t, L.s, *L.a, L.ta);
+      else
+        hipLaunchKernelGGL(pred_mfma_kernel<false>, dim3(c.G), dim3(kPT), L.lds_t, L.s, *L.a,
+                           L.ta);
+    }
+  };
+  PLaunch lau{s, &cands, lds_t, lds_p, &a, ta, precision == SVH_MIXED};
+  int pick = dflt;
+  if (ctx->tune.mode != 0 && cands.size() > 1) {
+    char key[160];
+    int lg = 0;
+    while ((int64_t(1) << (lg + 1)) <= nblk) ++lg;
+    std::snprintf(key, sizeof key, "pred_blocks:K%d:m%d:d%d:p%d:nb2^%d", K, mode, d, precision,
+                  lg);
+    pick = tune_choose(ctx, key, (int)cands.size(), dflt, &PLaunch::run, &lau);
+    hipLaunchKernelGGL(p_fill_int, dim3((nchunk + 255) / 256), dim3(256), 0, s, fail, 0,
+                       (int64_t)nchunk);
+  }
   int* hn = static_cast<int*>(ctx->stage.get(sizeof(int)));
   int rounds = 0;
   for (;;) {
     ctx->timers.begin(s, "predict_blocks");
-    if (!tiled)
-      hipLaunchKernelGGL(pred_blocks_kernel, dim3(G), dim3(kWave), lds, s, a);
-    else if (precision == SVH_MIXED)
-      hipLaunchKernelGGL(pred_mfma_kernel<true>, dim3(G), dim3(kPT), lds, s, a, ta);
-    else
-      hipLaunchKernelGGL(pred_mfma_kernel<false>, dim3(G), dim3(kPT), lds, s, a, ta);
+    PLaunch::run(&lau, pick);
     ctx->timers.end(s);
     hipLaunchKernelGGL(p_fill_int, dim3(1), dim3(1), 0, s, nfail, 0, (int64_t)1);
     hipLaunchKernelGGL(p_escalate, dim3((nchunk + 255) / 256), dim3(256), 0, s, level, fail,

@@ -148,6 +148,16 @@ struct KernelTimers {
   }
 };
 
+// Launch-schedule autotuner (tune.hip): per shape-bucket key, the index of the
+// fastest of a launch site's candidates, timed on the context stream.
+struct Autotune {
+  int mode = 0;        // 0: built-in defaults, 1: cached choices only, 2: also tune new keys
+  std::string path;    // cache file, "key index ms" lines ("" = in memory only)
+  bool loaded = false;
+  std::unordered_map<std::string, int> best;
+  std::unordered_map<std::string, double> best_ms;
+};
+
 struct MlpState;   // warm-start network (mlp.hip)
 
 struct svh_ctx {
@@ -160,6 +170,7 @@ struct svh_ctx {
   KernelTimers timers;
   int mm_rounds = 0;  // rounds of the last maximin call
   MlpState* mlp = nullptr;
+  Autotune tune;
   std::string last_error;
 };
 
@@ -178,5 +189,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
                          int* jitter_power_max);
 // mlp.hip
 void mlp_release(svh_ctx* ctx);
+// tune.hip: the candidate to launch at a tunable site.  Returns the cached
+// choice for `key`, or (mode 2) times launch(i) for every candidate (one
+// warm-up, median of three, HIP events on the context stream), caches and
+// returns the fastest; `dflt` when tuning is off.  launch(i) must be
+// repeatable (it may run several times before the real launch).
+int tune_choose(svh_ctx* ctx, const std::string& key, int ncand, int dflt,
+                void (*launch)(void*, int), void* user);
 // Jitter factors 10^p, p = -12..-4 (index 1..9; index 0 = no jitter).
 extern const double kJitterPow10[10];
