@@ -210,7 +210,8 @@ def block_flops(K, d, P, q):
             + 2 * K * (1 + q) + q * (q + 1) + 2 * q + K ** 2 + 2 * K * q + 2 * P * K)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ll_pass1_split_c2_traffic.json")
+def traffic_file(kernel):
+    return os.path.join(ROOT, "profiles", f"{kernel}_c2_traffic.json")
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -227,12 +228,12 @@ def calibrate_copy(nbytes=1 << 30):
                       "check": float(b[-1].item())}))
 
 
-def measured_traffic():
-    """HBM bytes per launch of the likelihood kernel from the committed
+def measured_traffic(kernel):
+    """HBM bytes per launch of the named likelihood kernel from the committed
     counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration),
     or None when no such summary exists."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(kernel)) as f:
             return float(json.load(f)["hbm_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
@@ -328,11 +329,14 @@ def main():
             res = st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
         t = time.perf_counter() - t0     # every call synchronises on return
         ctx.timing(False)
-        return res, t, ctx.timing_read("vecchia_pass1")
+        served = [k for k in ("ll_pass1_rows", "ll_pass1_split", "ll_pass1")
+                  if ctx.timing_read("kernel:" + k)[1] > 0]
+        return res, t, ctx.timing_read("vecchia_pass1"), served
 
     other = "mixed" if args.precision == "fp64" else "fp64"
-    (ll, g, F, beta), t_steps, (k_ms, k_cnt) = timed_evals(args.precision)
-    (ll_o, g_o, F_o, _), t_o, (k_ms_o, k_cnt_o) = timed_evals(other)
+    (ll, g, F, beta), t_steps, (k_ms, k_cnt), served = timed_evals(args.precision)
+    (ll_o, g_o, F_o, _), t_o, (k_ms_o, k_cnt_o), served_o = timed_evals(other)
+    kname = served[0] if len(served) == 1 else "+".join(served)
     ms_step = 1e3 * t_steps / args.steps
     g64, F64, ll64 = (g, F, ll) if args.precision == "fp64" else (g_o, F_o, ll_o)
     gmx, Fmx, llmx = (g_o, F_o, ll_o) if args.precision == "fp64" else (g, F, ll)
@@ -361,13 +365,13 @@ def main():
         "config": {"workload": f"C2 objective eval n={n} d={d} m_est={m} nu=2.5 trend=linear",
                    "global_batch": n, "seq_len": m + 1, "parallelism": "single-gpu",
                    "precision": args.precision},
-        "roofline": {"bound": "mfma", "kernel": "ll_pass1_split", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(),
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(kname),
                      "kernel_avg_ms": 1e3 * k_avg_s, "flops_per_launch": flops},
         "loglik": ll,
         other: {"evals_per_s": args.steps / t_o, "ms_per_step": 1e3 * t_o / args.steps,
-                "kernel_avg_ms": k_ms_o / max(k_cnt_o, 1)},
+                "kernel_avg_ms": k_ms_o / max(k_cnt_o, 1), "kernel": "+".join(served_o)},
         "mixed_vs_fp64_rel_err": {"loglik": abs(llmx - ll64) / abs(ll64),
                                   "grad": rel(gmx, g64), "fisher": rel(Fmx, F64)},
         "autotune": {k: {"candidate": v[0], "ms": v[1]} for k, v in ctx.autotune_choices().items()},

@@ -763,6 +763,339 @@ void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mi
     hipLaunchKernelGGL((ll_pass1_split<NU, false, false>), dim3(G), dim3(kWave), lds, st, a);
 }
 
+// ---------------------------------------------------------------- pass 1, rows in lanes
+// Two conditioning blocks per wave (K <= 32, d <= 10, q <= 11): half
+// h = lane >> 5 takes block 2w + h, lane r = lane & 31 owns row r of it (rows
+// K..31 are identity padding).  All per-row state lives in registers:
+//   * coordinates (DM >= d, zero padded) and the Sigma row, then the factor row;
+//   * the factorisation is right-looking in column-scaled form (unit lower L',
+//     pivots p_r = L_rr^2): step k broadcasts the current column k and the
+//     finished right-hand-side row z_k of [y Z] through LDS (one write, broadcast
+//     reads; a one-wave workgroup needs no barrier), so L^-1 [y Z] comes out
+//     of the factorisation itself;
+//   * t = L'^-T e_{K-1} by 31 half-wave sum reductions (rows hold their own
+//     column entries l'_rc), a = t / L_KK;
+//   * dS_p a row by row: each lane recomputes its row's distances and g(r)
+//     (no matrix of g is kept) -- sigma^2: L e_K - jitter a; ranges:
+//     -var sum_c g_rc (x_rl - x_cl)^2 a_c; nugget: var tau a;
+//   * W = L^-1 (dS a) by a column sweep, then the products W^T [W | L^-1 y |
+//     L^-1 Z] on the matrix cores, block by block (as in the split kernel).
+constexpr int kRQ = 11, kRC = 12;
+
+__device__ __forceinline__ double half_sum(double v) {
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
+  return v;
+}
+
+// LDS doubles per wave of the rows kernel: per half [cb 32 | bb 16 | ab 32 |
+// wb 16 | xs 32*dm] behind one base, then misc (16) and Mt (24 x 33)
+__host__ __device__ inline int rows_half_doubles(int dm) { return 96 + 32 * dm; }
+__host__ __device__ inline int rows_lds_doubles(int dm) {
+  return 2 * rows_half_doubles(dm) + 16 + 24 * 33;
+}
+
+template <int NU, bool MIX, int DM>
+__device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double* lds,
+                                          double (&accF)[4], double (&accX)[3]) {
+  const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
+  const int d = a.d, q = a.q, P = a.P;
+  const bool grad = a.need_grad != 0;
+  double* hb = lds + h * rows_half_doubles(DM);       // this half's region
+  double* cb = hb;                                    // column broadcast
+  double* bb = hb + 32;                               // right-hand-side row broadcast
+  double* ab = hb + 48;                               // a
+  double* wb = hb + 80;                               // derivative row broadcast
+  double* xs = hb + 96;                               // coordinates of this half's block
+  double* misc = lds + 2 * rows_half_doubles(DM);     // per half: [8h] fail, [8h+1..2] pivot
+  double* Mt = misc + 16;                             // 24 x 33 product operands
+  const int64_t b = 2 * pair + h;
+  const bool bvalid = b < a.nblk;
+  const int K = bvalid ? a.kcnt[b] + 1 : 1;
+  const int ch = bvalid ? a.chunk[b] : 0;
+  const int lev = bvalid ? a.level[ch] : 0;
+  const bool act = bvalid && r < K;
+  int pt = 0;
+  if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
+  double x[DM];
+#pragma unroll
+  for (int l = 0; l < DM; ++l) {
+    x[l] = (act && l < d) ? a.X[(int64_t)pt * d + l] * a.inv_lam[l] : 0.0;
+    xs[r * DM + l] = x[l];
+  }
+  double bv[1 + kRQ];
+  bv[0] = act ? a.y[pt] : 0.0;
+#pragma unroll
+  for (int c = 0; c < kRQ; ++c) bv[1 + c] = (act && c < q) ? a.Z[(int64_t)pt * q + c] : 0.0;
+  const double var = a.var, nug = a.nug;
+  double jit = 0.0;
+  if (lev > 0) {
+    double tr = 0.0;
+    for (int t = 0; t < K; ++t) tr += var * (1.0 + nug);
+    jit = kJitDev[lev] * (tr / K);
+  }
+  __syncthreads();
+  // ---- Sigma row r, lower part (upper part zero)
+  double S[32];
+#pragma unroll
+  for (int c = 0; c < 32; ++c) {
+    double v = 0.0;
+    if (c == r) {
+      v = act ? var * (1.0 + nug) + jit : 1.0;
+    } else if (c < r && act) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int l = 0; l < DM; ++l) {
+        const double t = x[l] - xs[c * DM + l];
+        s2 = fma(t, t, s2);
+      }
+      double M, g;
+      matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+      v = var * M;
+    }
+    S[c] = v;
+    __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
+  }
+  // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly
+  double pr = 1.0;
+  bool fail = false;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    cb[r] = S[k];
+    if (r == k) {
+#pragma unroll
+      for (int c = 0; c < 1 + kRQ; ++c) bb[c] = bv[c];
+    }
+    __syncthreads();
+    const double pk = cb[k];
+    fail |= (k < K) && !(pk > 0.0);
+    const double ik = rcp_nr(pk);
+    if (r == k) pr = pk;
+    const double lrk = S[k] * ik;
+#pragma unroll
+    for (int c = k + 1; c < 32; ++c) S[c] = fma(-lrk, cb[c], S[c]);
+    if (r > k) {
+      S[k] = lrk;
+#pragma unroll
+      for (int c = 0; c < 1 + kRQ; ++c) bv[c] = fma(-lrk, bb[c], bv[c]);
+    }
+    // keep each step's updates in place: left alone, the compiler defers them
+    // to the step that reads each entry and holds every broadcast column live
+#pragma unroll
+    for (int c = k + 1; c < 32; ++c) asm volatile("" : "+v"(S[c]));
+#pragma unroll
+    for (int c = 0; c < 1 + kRQ; ++c) asm volatile("" : "+v"(bv[c]));
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  const double dinv = rsqrt_nr(pr);   // 1 / L_rr
+#pragma unroll
+  for (int c = 0; c < 1 + kRQ; ++c) bv[c] *= dinv;   // rows of L^-1 [y Z]
+  // L_KK of this half (from lane K-1)
+  if (r == K - 1) {
+    misc[8 * h + 1] = pr;
+    misc[8 * h + 2] = dinv;
+  }
+  if (r == 0) misc[8 * h] = fail ? 1.0 : 0.0;
+  __syncthreads();
+  const double pK = misc[8 * h + 1], dK = misc[8 * h + 2];
+  double R[kRC];
+#pragma unroll
+  for (int c = 0; c < kRC; ++c) R[c] = 0.0;
+  if (grad) {
+    // ---- t = L'^-T e_{K-1}, a = t / L_KK
+    double t = (r == K - 1) ? 1.0 : 0.0;
+#pragma unroll
+    for (int c = 30; c >= 0; --c) {
+      const double s = half_sum((r > c) ? S[c] * t : 0.0);
+      if (r == c) t -= s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const double av = t * dK;
+    ab[r] = av;
+    __syncthreads();
+    // ---- dS_p a, row r
+    if (act) {
+#pragma unroll 2
+      for (int c = 0; c < 32; ++c) {
+        const double ac = ab[c];
+        double dl[DM], s2 = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          dl[l] = x[l] - xs[c * DM + l];
+          s2 = fma(dl[l], dl[l], s2);
+        }
+        double M, g;
+        matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+        const double w = g * ac;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) R[1 + l] = fma(w, dl[l] * dl[l], R[1 + l]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int l = 0; l < DM; ++l) R[1 + l] *= -var;
+      R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
+      R[kRC - 1] = var * nug * av;
+    }
+    // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
+#pragma unroll
+    for (int k = 0; k < 31; ++k) {
+      if (r == k) {
+#pragma unroll
+        for (int c = 0; c < kRC; ++c) wb[c] = R[c];
+      }
+      __syncthreads();
+      if (r > k) {
+        const double l = S[k];
+#pragma unroll
+        for (int c = 0; c < kRC; ++c) R[c] = fma(-l, wb[c], R[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < kRC; ++c) asm volatile("" : "+v"(R[c]));
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 0; c < kRC; ++c) R[c] *= dinv;
+  }
+  // ---- products and records, one block at a time on the whole wave
+  const int cyl = grad ? P : 0;
+  const int nX = q * q + q;
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const int64_t bh = 2 * pair + hh;
+    if (bh >= a.nblk) break;
+    const int Kh = a.kcnt[bh] + 1, km = Kh - 1;
+    const int chh = a.chunk[bh];
+    __syncthreads();
+    if (h == hh) {
+      if (grad) {
+        Mt[0 * 33 + r] = R[0];
+#pragma unroll
+        for (int l = 0; l < kSD; ++l)
+          if (l < d) Mt[(1 + l) * 33 + r] = R[1 + l];
+        Mt[(d + 1) * 33 + r] = R[kRC - 1];
+      }
+      Mt[cyl * 33 + r] = bv[0];
+#pragma unroll
+      for (int c = 0; c < kRQ; ++c)
+        if (c < q) Mt[(cyl + 1 + c) * 33 + r] = bv[1 + c];
+    }
+    __syncthreads();
+    const bool okh = misc[8 * hh] == 0.0;
+    if (!okh) {
+      if (lane == 0) atomicOr(a.fail + chh, 1);
+      continue;
+    }
+    double* rec = a.cache + bh * a.rec_stride;
+    for (int e = lane; e < 2 + q + (grad ? P : 0); e += kWave) {
+      double v;
+      if (e == 0)
+        v = Mt[cyl * 33 + km];
+      else if (e <= q)
+        v = Mt[(cyl + e) * 33 + km];
+      else if (e == q + 1)
+        v = 0.5 * log(misc[8 * hh + 1]);
+      else
+        v = Mt[(e - q - 2) * 33 + km];
+      rec[e] = v;
+    }
+    if (grad) {
+      typedef Mfma<MIX> MF;
+      const int li = lane & 15, lk = lane >> 4;
+      typename MF::acc_t D0 = MF::zero(), D1 = MF::zero();
+#pragma unroll 2
+      for (int st = 0; st < 8; ++st) {
+        const int i = 4 * st + lk;
+        const double w0 = (li < cyl + 1 + q) ? Mt[li * 33 + i] : 0.0;
+        const double w1 = (16 + li < cyl + 1 + q) ? Mt[(16 + li) * 33 + i] : 0.0;
+        D0 = MF::mma(w0, w0, D0);
+        D1 = MF::mma(w0, w1, D1);
+      }
+      const int N = cyl + 1 + q;
+      const double wKc0 = (li < N) ? Mt[li * 33 + km] : 0.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int p = MF::row(lk, v);
+        if (p < P) {
+          const double d0 = (double)D0[v], d1 = (double)D1[v];
+          if (li < P) {
+            accF[v] += d0 - 0.5 * Mt[p * 33 + km] * wKc0;
+          } else if (li == cyl) {
+            rec[2 + q + P + p] = d0;
+          } else if (li < N) {
+            rec[2 + q + 2 * P + p * q + (li - cyl - 1)] = d0;
+          }
+          if (16 + li < N) rec[2 + q + 2 * P + p * q + (16 + li - cyl - 1)] = d1;
+        }
+      }
+    }
+    for (int e = lane; e < nX; e += kWave) {
+      double v;
+      if (e < q * q) {
+        const int c1 = e / q, c2 = e - c1 * q;
+        v = Mt[(cyl + 1 + c1) * 33 + km] * Mt[(cyl + 1 + c2) * 33 + km];
+      } else {
+        const int c = e - q * q;
+        v = Mt[(cyl + 1 + c) * 33 + km] * Mt[cyl * 33 + km];
+      }
+      if (e < kWave)
+        accX[0] += v;
+      else if (e < 2 * kWave)
+        accX[1] += v;
+      else
+        accX[2] += v;
+    }
+  }
+}
+
+template <int NU, bool MIX, int DM, int OCC>
+__global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  const int P = a.P, q = a.q;
+  const bool grad = a.need_grad != 0;
+  const int nX = q * q + q;
+  double accF[4] = {0.0, 0.0, 0.0, 0.0};
+  double accX[3] = {0.0, 0.0, 0.0};
+  const int64_t npair = (a.nblk + 1) / 2;
+  for (int64_t pp = blockIdx.x; pp < npair; pp += gridDim.x)
+    rows_pair<NU, MIX, DM>(a, pp, lds, accF, accX);
+  const int NACC = grad ? P * P + q * q + q : q * q + q;
+  const int X0 = grad ? P * P : 0;
+  double* part = a.partial + (int64_t)blockIdx.x * NACC;
+  if (grad) write_fisher<MIX>(part, accF, P, lane);
+  for (int e = lane; e < nX; e += kWave)
+    part[X0 + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
+}
+
+// occ: target waves per SIMD (2: 256 VGPRs, 3: 168 VGPRs with some spilling)
+template <int NU, int DM>
+void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed, int occ) {
+  if (occ == 3) {
+    if (mixed)
+      hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 3>), dim3(G), dim3(kWave), lds, st, a);
+    else
+      hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 3>), dim3(G), dim3(kWave), lds, st, a);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
+    else
+      hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
+  }
+}
+
+template <int NU>
+void launch_rows_nu(hipStream_t st, int G, int dm, const LLArgs& a, bool mixed, int occ) {
+  const size_t lds = (size_t)rows_lds_doubles(dm) * sizeof(double);
+  if (dm == 4)
+    launch_rows_nu_dm<NU, 4>(st, G, lds, a, mixed, occ);
+  else if (dm == 8)
+    launch_rows_nu_dm<NU, 8>(st, G, lds, a, mixed, occ);
+  else
+    launch_rows_nu_dm<NU, 10>(st, G, lds, a, mixed, occ);
+}
+
 // ---------------------------------------------------------------- reduce 1
 // (a) one workgroup per accumulator entry: fixed-order strided sums of the
 //     pass-1 partials, then an LDS tree;
@@ -1484,28 +1817,41 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   const int NC = need_grad ? 2 + q + 2 * P + P * q : 2 + q;
   const int NV = need_grad ? 1 + P : 1;
   // split-row kernel (one block per wave, rows in registers) when the block fits
+  // SVH_LL_KERNEL = rows | split | lds forces a kernel variant (A/B timing)
   const char* kenv = std::getenv("SVH_LL_KERNEL");
-  const bool use_split = !(kenv && std::string(kenv) == "lds") && vs.mmax + 1 <= kSK &&
-                         d <= kSD && q <= kSQ;
+  const std::string kv = kenv ? kenv : "";
+  const bool fits32 = vs.mmax + 1 <= kSK && d <= kSD && q <= kSQ;
+  const bool use_split = fits32 && kv != "lds";
+  const bool use_rows = fits32 && kv != "lds" && kv != "split";
   const size_t lds_split = (size_t)split_lds_doubles(d) * sizeof(double);
-  // launch-schedule candidates for the autotuner: split kernel at 2..32
-  // workgroups per CU, then the general LDS kernel; defaults as fixed before
+  // launch-schedule candidates for the autotuner: the rows-in-lanes kernel at
+  // two occupancies and several grid sizes, the split-row kernel, and the
+  // general LDS kernel for blocks beyond 32 rows; the default comes first in
+  // the order rows > split > LDS
   const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+  // kind 0: general LDS kernel, 1: split-row kernel, 2: rows-in-lanes kernel
   struct Cand {
-    bool split;
+    int kind;
     int G;
   };
   std::vector<Cand> cands;
   int dflt = 0;
+  const int64_t npair = (vs.nblk + 1) / 2;
+  if (use_rows) {
+    // kind 2: rows kernel at 2 waves per SIMD, kind 3: at 3 waves per SIMD
+    for (int f : {4, 16}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
+    dflt = (int)cands.size();
+    cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)8 * ncu)});
+    for (int f : {12, 24}) cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
+  }
   if (use_split) {
-    for (int f : {2, 4, 8, 16, 32}) cands.push_back({true, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
+    for (int f : {8, 32}) cands.push_back({1, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
+    if (!use_rows) dflt = (int)cands.size() - 1;
+  }
+  if (kv == "lds" || !fits32) {
+    for (int f : {2, 4, 16}) cands.push_back({0, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
     dflt = (int)cands.size();
-    cands.push_back({true, (int)std::min<int64_t>(vs.nblk, 8192)});
-    if (!(vs.mmax + 1 > kMaxKP)) cands.push_back({false, (int)std::min<int64_t>(vs.nblk, 2048)});
-  } else {
-    for (int f : {2, 4, 16}) cands.push_back({false, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
-    dflt = (int)cands.size();
-    cands.push_back({false, (int)std::min<int64_t>(vs.nblk, 2048)});
+    cands.push_back({0, (int)std::min<int64_t>(vs.nblk, 2048)});
   }
   int Gmax = 1;
   for (const Cand& c : cands) Gmax = std::max(Gmax, c.G);
@@ -1561,7 +1907,14 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     static void run(void* u, int i) {
       const Launch& L = *static_cast<Launch*>(u);
       const Cand& c = (*L.cands)[i];
-      if (c.split) {
+      if (c.kind >= 2) {
+        const int dm = L.a->d <= 4 ? 4 : (L.a->d <= 8 ? 8 : 10);
+        switch (L.nu_code) {
+          case 0: launch_rows_nu<0>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
+          case 1: launch_rows_nu<1>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
+          default: launch_rows_nu<2>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
+        }
+      } else if (c.kind == 1) {
         switch (L.nu_code) {
           case 0: launch_split_nu<0>(L.s, c.G, L.lds_split, *L.a, L.mixed); break;
           case 1: launch_split_nu<1>(L.s, c.G, L.lds_split, *L.a, L.mixed); break;
@@ -1577,6 +1930,10 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     }
   };
   Launch lau{s, &cands, nu_code, KP, lds_split, lds, &a, precision == SVH_MIXED};
+  if (a.prof) {   // per-phase clocks exist in the split-row kernel only
+    for (int i = 0; i < (int)cands.size(); ++i)
+      if (cands[i].kind == 1) dflt = i;
+  }
   if (ctx->tune.mode != 0 && !a.prof) {
     char key[160];
     int lg = 0;
@@ -1593,6 +1950,9 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     cands.swap(one);
   }
 
+  static const char* const kKindName[4] = {"kernel:ll_pass1", "kernel:ll_pass1_split",
+                                           "kernel:ll_pass1_rows", "kernel:ll_pass1_rows"};
+  ctx->timers.count(kKindName[cands[0].kind]);
   int rounds = 0;
   for (;;) {
     ctx->timers.begin(s, "vecchia_pass1");

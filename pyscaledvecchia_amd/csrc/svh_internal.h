@@ -120,6 +120,10 @@ struct KernelTimers {
     SVH_HIP(hipEventRecord(p.a, s));
     pending.push_back(p);
   }
+  // a count without a clock (which of several kernels served a call)
+  void count(const char* name) {
+    if (enabled) acc[name].count += 1;
+  }
   void end(hipStream_t s) {
     if (!enabled || pending.empty()) return;
     SVH_HIP(hipEventRecord(pending.back().b, s));
