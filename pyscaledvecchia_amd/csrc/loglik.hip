@@ -795,9 +795,12 @@ __host__ __device__ inline int rows_lds_doubles(int dm) {
   return 2 * rows_half_doubles(dm) + 16 + 24 * 33;
 }
 
-template <int NU, bool MIX, int DM>
+template <int NU, bool MIX, int DM, bool PROF>
 __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double* lds,
-                                          double (&accF)[4], double (&accX)[3]) {
+                                          double (&accF)[4], double (&accX)[3],
+                                          unsigned long long (&ph)[8]) {
+  unsigned long long ph_t = 0;
+  if constexpr (PROF) ph_t = __builtin_amdgcn_s_memtime();
   const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
@@ -835,6 +838,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
     jit = kJitDev[lev] * (tr / K);
   }
   __syncthreads();
+  SVH_PHASE(0)
   // ---- Sigma row r, lower part (upper part zero)
   double S[32];
 #pragma unroll
@@ -856,6 +860,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
     S[c] = v;
     __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
   }
+  SVH_PHASE(1)
   // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly
   double pr = 1.0;
   bool fail = false;
@@ -899,6 +904,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
   if (r == 0) misc[8 * h] = fail ? 1.0 : 0.0;
   __syncthreads();
   const double pK = misc[8 * h + 1], dK = misc[8 * h + 2];
+  SVH_PHASE(2)
   double R[kRC];
 #pragma unroll
   for (int c = 0; c < kRC; ++c) R[c] = 0.0;
@@ -914,6 +920,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
     const double av = t * dK;
     ab[r] = av;
     __syncthreads();
+    SVH_PHASE(3)
     // ---- dS_p a, row r
     if (act) {
 #pragma unroll 2
@@ -937,6 +944,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
       R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
       R[kRC - 1] = var * nug * av;
     }
+    SVH_PHASE(4)
     // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
 #pragma unroll
     for (int k = 0; k < 31; ++k) {
@@ -957,6 +965,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
     }
 #pragma unroll
     for (int c = 0; c < kRC; ++c) R[c] *= dinv;
+    SVH_PHASE(5)
   }
   // ---- products and records, one block at a time on the whole wave
   const int cyl = grad ? P : 0;
@@ -1047,9 +1056,10 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
         accX[2] += v;
     }
   }
+  SVH_PHASE(6)
 }
 
-template <int NU, bool MIX, int DM, int OCC>
+template <int NU, bool MIX, int DM, int OCC, bool PROF = false>
 __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
@@ -1059,8 +1069,11 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   const int64_t npair = (a.nblk + 1) / 2;
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t pp = blockIdx.x; pp < npair; pp += gridDim.x)
-    rows_pair<NU, MIX, DM>(a, pp, lds, accF, accX);
+    rows_pair<NU, MIX, DM, PROF>(a, pp, lds, accF, accX, ph);
+  if (PROF && lane == 0)
+    for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   const int NACC = grad ? P * P + q * q + q : q * q + q;
   const int X0 = grad ? P * P : 0;
   double* part = a.partial + (int64_t)blockIdx.x * NACC;
@@ -1072,6 +1085,13 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
 // occ: target waves per SIMD (2: 256 VGPRs, 3: 168 VGPRs with some spilling)
 template <int NU, int DM>
 void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed, int occ) {
+  if constexpr (NU == 2 && DM == 8) {   // phase clocks: FP64, nu = 5/2, d in 5..8
+    if (a.prof) {
+      hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 2, true>), dim3(G), dim3(kWave), lds, st,
+                         a);
+      return;
+    }
+  }
   if (occ == 3) {
     if (mixed)
       hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 3>), dim3(G), dim3(kWave), lds, st, a);
@@ -1886,7 +1906,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.rec_stride = NC;
   a.prof = nullptr;
   const char* penv = std::getenv("SVH_PHASE_PROFILE");
-  if (penv && use_split) {
+  if (penv && (use_split || use_rows)) {
     a.prof = ctx->arena.get_t<unsigned long long>("v.prof", 8);
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, (int*)a.prof, 0, (int64_t)16);
   }
@@ -1930,7 +1950,10 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     }
   };
   Launch lau{s, &cands, nu_code, KP, lds_split, lds, &a, precision == SVH_MIXED};
-  if (a.prof) {   // per-phase clocks exist in the split-row kernel only
+  // per-phase clocks exist in the split-row kernel and in the FP64 rows kernel
+  // for nu = 5/2 with 5..8 dimensions
+  const bool rows_prof = use_rows && nu_code == 2 && d > 4 && d <= 8 && precision != SVH_MIXED;
+  if (a.prof && !rows_prof) {
     for (int i = 0; i < (int)cands.size(); ++i)
       if (cands[i].kind == 1) dflt = i;
   }
@@ -1998,8 +2021,13 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   if (a.prof) {
     unsigned long long hp[8];
     SVH_HIP(hipMemcpy(hp, a.prof, 7 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    std::fprintf(stderr, "[svh phase clocks] gather %llu pairs %llu chol %llu back %llu rhs %llu "
-                 "solve %llu dots %llu\n", hp[0], hp[1], hp[2], hp[3], hp[4], hp[5], hp[6]);
+    if (cands[0].kind >= 2)
+      std::fprintf(stderr, "[svh phase clocks] rows: gather %llu sigma %llu chol %llu a %llu "
+                   "dsa %llu wsolve %llu products %llu\n", hp[0], hp[1], hp[2], hp[3], hp[4],
+                   hp[5], hp[6]);
+    else
+      std::fprintf(stderr, "[svh phase clocks] gather %llu pairs %llu chol %llu back %llu rhs %llu "
+                   "solve %llu dots %llu\n", hp[0], hp[1], hp[2], hp[3], hp[4], hp[5], hp[6]);
   }
 }
 

@@ -11,6 +11,8 @@
 #include "svh_internal.h"
 
 #include <climits>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace {
@@ -49,59 +51,6 @@ __device__ __forceinline__ void wave_argmax(double& k, int& i) {
   }
 }
 
-__device__ __forceinline__ void wave_argmax3(double& k, int& i, int& e) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    double k2 = __shfl_xor(k, off, kWave);
-    int i2 = __shfl_xor(i, off, kWave);
-    int e2 = __shfl_xor(e, off, kWave);
-    if (mm_greater(k2, i2, k, i)) {
-      k = k2;
-      i = i2;
-      e = e2;
-    }
-  }
-}
-
-// Full-wave maximum of an unsigned value with DPP moves (result in every lane):
-// quad swaps, row shifts with bank masks, row broadcasts; lane 63 ends with
-// the maximum, which readlane broadcasts.  Every lane must be active.
-__device__ __forceinline__ unsigned dpp_wave_max(unsigned v) {
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xE, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xC, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));
-  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-// Order-preserving map of a double to an unsigned 64-bit key.
-__device__ __forceinline__ unsigned long long ord_key(double x) {
-  unsigned long long b = (unsigned long long)__double_as_longlong(x);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-// Exact wave arg-max of (key desc, idx asc) by three DPP reductions: high
-// word of the key, low word among the tied, then the smallest index among
-// exact ties.  Returns the winning idx and the winner lane's `e`.
-__device__ __forceinline__ void wave_argmax_dpp(double k, int i, int e, double& bk, int& bi,
-                                                int& be) {
-  const unsigned long long u = ord_key(k);
-  const unsigned hi = (unsigned)(u >> 32), lo = (unsigned)u;
-  const unsigned m1 = dpp_wave_max(hi);
-  const unsigned m2 = dpp_wave_max(hi == m1 ? lo : 0u);
-  const bool tie = (hi == m1) && (lo == m2);
-  const unsigned mi = dpp_wave_max(tie ? ~(unsigned)i : 0u);
-  bi = (int)~mi;
-  const unsigned long long win = __ballot(tie && i == bi);
-  const int wl = (int)__ffsll((long long)win) - 1;
-  be = __builtin_amdgcn_readlane(e, wl);
-  const unsigned long long ku = ((unsigned long long)m1 << 32) | m2;
-  const unsigned long long kb = (ku >> 63) ? (ku & 0x7fffffffffffffffull) : ~ku;
-  bk = __longlong_as_double((long long)kb);
-}
-
 // ------------------------------------------------------------------ transpose
 // XT[l*n + j] = X[j*d + l] * (scale ? 1 : 1) / lam[l] (lam may be NULL).
 __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
@@ -128,12 +77,19 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 //   mm_select (one workgroup, one pool slot per thread): V = the largest
 //             published "next" element over all slices is an upper bound (in
 //             that order) on every point that is not a published candidate.
-//             Candidates above V form a pool; selection repeats "take the pool
-//             maximum, lower pool distances with the new point" while the
-//             maximum stays above V (the first pick of a round is the global
-//             maximum and always valid).  Each pick is exactly the serial
-//             algorithm's next step: every point outside the pool is at or
-//             below V and distances only fall.
+//             Candidates above V form a pool, kept sorted by (dist desc, index
+//             asc).  A step accepts the longest prefix c_1..c_L of the sorted
+//             pool (at most kWin entries) in which no c_i is lowered by an
+//             earlier c_j (squared distance below c_i's own): c_1 is the
+//             serial algorithm's next pick, and by induction so is each c_i --
+//             its distance is unchanged by c_1..c_{i-1}, every later pool entry
+//             was at or below it and distances only fall, and every point
+//             outside the pool is at or below V.  The step then lowers the
+//             remaining pool with the L accepted points (entries falling to V
+//             or below leave the pool) and re-sorts it.  The round ends when
+//             the pool is empty.  On well-spread inputs most steps accept the
+//             whole window, so the serial chain of single picks becomes a few
+//             wide steps per round.
 // Coordinates are held in DM >= d registers per point with the extra
 // coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
 // the squared distances are bit-identical to the d-term sums.
@@ -143,6 +99,7 @@ constexpr int kPool = 256;            // pool slots = select threads
 constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
 constexpr int kMMD = 12;              // max d of the maximin kernels
+constexpr int kWin = 64;              // prefix window of one selection step (one wave)
 
 struct MMState {
   int count;  // points placed in `order`
@@ -158,42 +115,22 @@ struct MMDesc {
   int n, d, first, slice0, nslice, pad;
 };
 
-// Wave arg-max of (key desc, idx asc), exact.  Fast path: one DPP max over
-// the high words of the order-preserving keys and a ballot; the low words and
-// the index are compared only when several lanes share the maximal high word.
-// Returns the winner's key, idx and `e`.  Every lane must be active.
-__device__ __forceinline__ void wave_argmax_fast(double k, int i, int e, double& bk, int& bi,
-                                                 int& be) {
-  const unsigned long long u = ord_key(k);
-  const unsigned hi = (unsigned)(u >> 32), lo = (unsigned)u;
-  const unsigned m1 = dpp_wave_max(hi);
-  unsigned long long ball = __ballot(hi == m1);
-  if (__popcll(ball) != 1) {
-    const unsigned m2 = dpp_wave_max(hi == m1 ? lo : 0u);
-    const bool tie = (hi == m1) && (lo == m2);
-    ball = __ballot(tie);
-    if (__popcll(ball) != 1) {
-      const unsigned mi = dpp_wave_max(tie ? ~(unsigned)i : 0u);
-      ball = __ballot(tie && i == (int)~mi);
-    }
-  }
-  const int wl = (int)__ffsll((long long)ball) - 1;
-  const long long kb = __double_as_longlong(k);
-  const int klo = __builtin_amdgcn_readlane((int)(kb & 0xffffffffll), wl);
-  const int khi = __builtin_amdgcn_readlane((int)(kb >> 32), wl);
-  bk = __longlong_as_double(((long long)khi << 32) | (unsigned)klo);
-  bi = __builtin_amdgcn_readlane(i, wl);
-  be = __builtin_amdgcn_readlane(e, wl);
-}
+// One workgroup per slice, kApplyParts threads per point: part p applies the
+// accepted points p, p + kApplyParts, ...; the parts' minima are combined in
+// LDS (a minimum does not depend on the order of its terms).
+constexpr int kApplyParts = 4;
 
 template <int DM>
-__global__ __launch_bounds__(kSlice) void mm_apply_kernel(
+__global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     double* __restrict__ s2all, double* __restrict__ distall, const int* __restrict__ accall,
     const MMState* __restrict__ stall, double* __restrict__ cand_keyall,
     int* __restrict__ cand_idxall) {
+  constexpr int NT = kSlice * kApplyParts;
   __shared__ double accX[kAccCap * DM];
   __shared__ int accI[kAccCap];
+  __shared__ double pmin[kApplyParts - 1][kSlice];
+  __shared__ int ppick[kApplyParts - 1][kSlice];
   __shared__ double wk[kSlice / kWave];
   __shared__ int wi[kSlice / kWave];
   const MMDesc D = desc[blockIdx.y];
@@ -208,56 +145,67 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
   double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
   int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
   const int A = st->nacc;
-  for (int t = threadIdx.x; t < A; t += kSlice) accI[t] = acc[t];
+  const int tid = threadIdx.x, part = tid / kSlice, pt = tid - part * kSlice;
+  for (int t = tid; t < A; t += NT) accI[t] = acc[t];
   __syncthreads();
-  for (int t = threadIdx.x; t < A * DM; t += kSlice) {
+  for (int t = tid; t < A * DM; t += NT) {
     const int a = t / DM, l = t - a * DM;
     accX[t] = (l < d) ? XT[(int64_t)l * n + accI[a]] : 0.0;
   }
   __syncthreads();
-  const int i = blockIdx.x * kSlice + threadIdx.x;
-  double di = -1.0;
-  if (i < n) {
-    di = dist[i];
-    if (di >= 0.0 && A > 0) {
-      double xi[DM];
+  const int i = blockIdx.x * kSlice + pt;
+  double di = (i < n) ? dist[i] : -1.0;
+  const bool active = i < n && di >= 0.0 && A > 0;
+  double smin = INFINITY;
+  bool picked = false;
+  if (active) {
+    double xi[DM];
 #pragma unroll
-      for (int l = 0; l < DM; ++l) xi[l] = (l < d) ? XT[(int64_t)l * n + i] : 0.0;
-      double s = s2[i];
-      bool changed = false, picked = false;
+    for (int l = 0; l < DM; ++l) xi[l] = (l < d) ? XT[(int64_t)l * n + i] : 0.0;
 #pragma unroll 2
-      for (int a = 0; a < A; ++a) {
-        picked |= (accI[a] == i);
-        double t = 0.0;
-        const double* xa = accX + a * DM;
+    for (int a = part; a < A; a += kApplyParts) {
+      picked |= (accI[a] == i);
+      double t = 0.0;
+      const double* xa = accX + a * DM;
 #pragma unroll
-        for (int l = 0; l < DM; ++l) t = sq_step(t, xi[l], xa[l]);
-        if (t < s) {
-          s = t;
-          changed = true;
-        }
-      }
-      if (picked) {
-        di = -1.0;
-        dist[i] = -1.0;
-      } else if (changed) {
-        s2[i] = s;
-        di = sqrt(s);
-        dist[i] = di;
-      }
+      for (int l = 0; l < DM; ++l) t = sq_step(t, xi[l], xa[l]);
+      smin = t < smin ? t : smin;
+    }
+    if (part > 0) {
+      pmin[part - 1][pt] = smin;
+      ppick[part - 1][pt] = picked ? 1 : 0;
     }
   }
-  // top-(kSliceTop+1) of this slice by (dist desc, index asc)
+  __syncthreads();
+  if (part == 0 && active) {
+#pragma unroll
+    for (int p = 0; p < kApplyParts - 1; ++p) {
+      const double v = pmin[p][pt];
+      smin = v < smin ? v : smin;
+      picked = picked || ppick[p][pt] != 0;
+    }
+    if (picked) {
+      di = -1.0;
+      dist[i] = -1.0;
+    } else if (smin < s2[i]) {
+      s2[i] = smin;
+      di = sqrt(smin);
+      dist[i] = di;
+    }
+  }
+  // top-(kSliceTop+1) of this slice by (dist desc, index asc), on part 0
   double k = (i < n) ? di : -1.0;
   int ki = (i < n) ? i : INT_MAX;
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
   for (int r = 0; r <= kSliceTop; ++r) {
-    double bk = k;
-    int bi = ki;
-    wave_argmax(bk, bi);
-    if (lane == 0) {
-      wk[wid] = bk;
-      wi[wid] = bi;
+    if (part == 0) {   // waves 0 .. kSlice/kWave - 1
+      double bk = k;
+      int bi = ki;
+      wave_argmax(bk, bi);
+      if (lane == 0) {
+        wk[wid] = bk;
+        wi[wid] = bi;
+      }
     }
     __syncthreads();
     double gk = wk[0];
@@ -267,7 +215,7 @@ __global__ __launch_bounds__(kSlice) void mm_apply_kernel(
         gk = wk[w];
         gi = wi[w];
       }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       cand_key[blockIdx.x * (kSliceTop + 1) + r] = gk;
       cand_idx[blockIdx.x * (kSliceTop + 1) + r] = gi;
     }
@@ -301,12 +249,97 @@ __device__ __forceinline__ void block_argmax(double& k, int& i, double* rk, int*
     }
 }
 
+// Lane exchange with lane ^ M inside a wave, for a compile-time M: DPP quad
+// permutes for 1 and 2, swizzles within 32 lanes for 4..16, a backwards
+// permute for 32.
+template <int M>
+__device__ __forceinline__ int xchg(int v) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  } else if constexpr (M < 32) {
+    return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);            // lane ^ M within 32
+  } else {
+    return __builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ M) << 2), v);
+  }
+}
+
+template <int M>
+__device__ __forceinline__ double xchg(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = xchg<M>((int)(b & 0xffffffffll)), hi = xchg<M>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One bitonic compare-exchange between position tid and tid ^ STRIDE inside
+// blocks of SIZE, (key desc, idx asc) order in the final block.  Both partners
+// evaluate the same strict comparison without branches, so a pair equal in
+// (key, idx) stays as it is.
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_step(double& k, int& i, int& s, double k2, int i2,
+                                             int s2) {
+  const int tid = threadIdx.x;
+  const bool desc_ = (tid & SIZE) == 0, lo = (tid & STRIDE) == 0;
+  const bool eq = k2 == k;
+  const bool pg = (k2 > k) | (eq & (i2 < i));   // partner ahead in the order
+  const bool pl = (k2 < k) | (eq & (i2 > i));   // partner behind
+  const bool take = (desc_ == lo) ? pg : pl;
+  k = take ? k2 : k;
+  i = take ? i2 : i;
+  s = take ? s2 : s;
+}
+
+template <int NT, int SIZE, int STRIDE>
+__device__ __forceinline__ void pool_sort_stage(double& k, int& i, int& s, double* bk, int* bi,
+                                                int* bs) {
+  if constexpr (STRIDE >= kWave) {
+    const int tid = threadIdx.x;
+    bk[tid] = k;
+    bi[tid] = i;
+    bs[tid] = s;
+    __syncthreads();
+    const double k2 = bk[tid ^ STRIDE];
+    const int i2 = bi[tid ^ STRIDE], s2 = bs[tid ^ STRIDE];
+    __syncthreads();
+    bitonic_step<SIZE, STRIDE>(k, i, s, k2, i2, s2);
+  } else {
+    bitonic_step<SIZE, STRIDE>(k, i, s, xchg<STRIDE>(k), xchg<STRIDE>(i), xchg<STRIDE>(s));
+  }
+  if constexpr (STRIDE > 1) pool_sort_stage<NT, SIZE, STRIDE / 2>(k, i, s, bk, bi, bs);
+}
+
+template <int NT, int SIZE>
+__device__ __forceinline__ void pool_sort_size(double& k, int& i, int& s, double* bk, int* bi,
+                                               int* bs) {
+  pool_sort_stage<NT, SIZE, SIZE / 2>(k, i, s, bk, bi, bs);
+  if constexpr (SIZE < NT) pool_sort_size<NT, SIZE * 2>(k, i, s, bk, bi, bs);
+}
+
+// Bitonic sort of NT (key, idx, slot) entries, one per thread (position =
+// thread), into (key desc, idx asc) order; every stage is unrolled.  Strides
+// of a wave or more exchange through LDS (bk, bi, bs hold NT entries).
+template <int NT>
+__device__ __forceinline__ void pool_sort(double& k, int& i, int& s, double* bk, int* bi, int* bs) {
+  pool_sort_size<NT, 2>(k, i, s, bk, bi, bs);
+}
+
 template <int DM>
 __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     const double* __restrict__ s2all, const double* __restrict__ cand_keyall,
     const int* __restrict__ cand_idxall, int* __restrict__ accall, MMState* __restrict__ stall,
-    int64_t* __restrict__ orderall, int sortcap) {
+    int64_t* __restrict__ orderall, int sortcap, unsigned long long* __restrict__ prof) {
+  // prof (profiling runs only): shader clocks of thread 0 per phase -- set-up,
+  // sorts, window checks, pool updates -- and the number of steps
+  unsigned long long ph[4] = {0, 0, 0, 0}, steps = 0;
+  unsigned long long ph_t = prof ? __builtin_amdgcn_s_memtime() : 0;
+#define MM_PHASE(k)                                            \
+  if (prof) {                                                  \
+    const unsigned long long t_ = __builtin_amdgcn_s_memtime(); \
+    ph[k] += t_ - ph_t;                                        \
+    ph_t = t_;                                                 \
+  }
   constexpr int NT = kPool, NW = NT / kWave;
   const MMDesc D = desc[blockIdx.x];
   const int n = D.n, d = D.d, nslice = D.nslice;
@@ -325,8 +358,13 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   __shared__ double pc[NT * DM];
   __shared__ double rk[NW];
   __shared__ int ri[NW], wcnt[NW];
-  __shared__ double bks[2][NW];
-  __shared__ int bis[2][NW], bes[2][NW];
+  // sorted pool: key, index and slot per position; squared distance per slot
+  __shared__ double qk[NT], pss[NT];
+  __shared__ int qi[NT], qs[NT];
+  __shared__ unsigned long long wlow[NW];
+  __shared__ int stepL;
+  __shared__ double wc[kWin * DM], ws2[kWin];   // window coordinates, squared distances
+  __shared__ int accf[NT];                      // slot accepted in this step
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
   if (count0 >= n) {
@@ -443,56 +481,105 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     px[l] = (l < d && tid < nsurv) ? XT[(int64_t)l * n + pi] : 0.0;
     pc[tid * DM + l] = px[l];
   }
-  __syncthreads();
-  int nacc = 0, par = 0;
+  accf[tid] = 0;
+  int nacc = 0;
   const int room = n - count0;
-  while (nacc < kAccCap && nacc < room) {
-    double wk;
-    int wi, we;
-    wave_argmax_fast(pv, pi, tid, wk, wi, we);
-    if (lane == 0) {
-      bks[par][wid] = wk;
-      bis[par][wid] = wi;
-      bes[par][wid] = we;
+  const int cap = room < kAccCap ? room : kAccCap;
+  static_assert(kWin * 4 == NT && kWin == kWave, "window layout");
+  MM_PHASE(0)
+  while (nacc < cap) {
+    ++steps;
+    // sort the pool by (dist desc, index asc); slots travel with their keys
+    pss[tid] = ps;
+    {
+      double k = pv;
+      int i = pi, sl = tid;
+      pool_sort<NT>(k, i, sl, qk, qi, qs);
+      qk[tid] = k;
+      qi[tid] = i;
+      qs[tid] = sl;
     }
     __syncthreads();
-    double gk = bks[par][0];
-    int gi = bis[par][0], ge = bes[par][0];
-#pragma unroll
-    for (int w = 1; w < NW; ++w)
-      if (mm_greater(bks[par][w], bis[par][w], gk, gi)) {
-        gk = bks[par][w];
-        gi = bis[par][w];
-        ge = bes[par][w];
-      }
-    par ^= 1;
-    if (gk < 0.0) break;
-    if (nacc > 0 && !mm_greater(gk, gi, Vk, Vi)) break;
-    if (tid == 0) {
-      acc[nacc] = gi;
-      order[count0 + nacc] = gi;
+    // the window's coordinates and squared distances, contiguous by position
+    for (int e = tid; e < kWin * DM; e += NT) {
+      const int w = e / DM;
+      wc[e] = pc[qs[w] * DM + (e - w * DM)];
     }
-    ++nacc;
-    if (pi == gi) {
-      pv = -3.0;
-    } else if (pv >= 0.0 && mm_greater(pv, pi, Vk, Vi)) {
-      // only entries still above V can be picked later in this round
-      double s = 0.0;
+    if (tid < kWin) ws2[tid] = pss[qs[tid]];
+    __syncthreads();
+    MM_PHASE(1)
+    // window conflicts: is c_w lowered by an earlier c_j?  Four threads per
+    // window entry (consecutive lanes), each taking every fourth j.
+    const int w = tid >> 2, wsub = tid & 3;
+    bool low = false;
+    if (w > 0 && qk[w] >= 0.0) {
+      const double s_w = ws2[w];
+      double xw[DM];
 #pragma unroll
-      for (int l = 0; l < DM; ++l) s = sq_step(s, px[l], pc[ge * DM + l]);
-      if (s < ps) {
-        ps = s;
-        pv = sqrt(s);
+      for (int l = 0; l < DM; ++l) xw[l] = wc[w * DM + l];
+      for (int j = wsub; j < w; j += 4) {
+        double t = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) t = sq_step(t, xw[l], wc[j * DM + l]);
+        low = low || (t < s_w);
       }
-    } else {
-      pv = -3.0;
     }
+    const unsigned long long lb = __ballot(low);
+    if (lane == 0) wlow[wid] = lb;
+    __syncthreads();
+    if (wid == 0) {
+      // window entry `lane`: bad when lowered or out of the pool
+      const unsigned nib = (unsigned)(wlow[lane >> 4] >> (4 * (lane & 15))) & 0xFu;
+      const bool bad = (lane > 0 && nib != 0u) || !(qk[lane] >= 0.0);
+      const unsigned long long bb = __ballot(bad);
+      int L = bb ? (int)__ffsll((long long)bb) - 1 : kWin;
+      if (L > cap - nacc) L = cap - nacc;
+      if (lane == 0) stepL = L;
+    }
+    __syncthreads();
+    const int L = stepL;
+    MM_PHASE(2)
+    if (L == 0) break;   // pool empty: every remaining entry is at or below V
+    if (tid < L) {
+      acc[nacc + tid] = qi[tid];
+      order[count0 + nacc + tid] = qi[tid];
+      accf[qs[tid]] = 1;
+    }
+    __syncthreads();
+    // lower the rest of the pool with the accepted points (window positions
+    // 0..L-1); entries that fall to V or below cannot be picked this round
+    if (accf[tid]) {
+      accf[tid] = 0;
+      pv = -3.0;
+    } else if (pv >= 0.0) {
+      double smin = ps;
+#pragma unroll 4
+      for (int j = 0; j < L; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) t = sq_step(t, px[l], wc[j * DM + l]);
+        smin = t < smin ? t : smin;
+      }
+      if (smin < ps) {
+        ps = smin;
+        pv = sqrt(ps);
+        if (!mm_greater(pv, pi, Vk, Vi)) pv = -3.0;
+      }
+    }
+    nacc += L;
+    __syncthreads();   // wc / qs are rewritten by the next step
+    MM_PHASE(3)
   }
   if (tid == 0) {
     st->nacc = nacc;
     st->count = count0 + nacc;
     st->rounds += 1;
+    if (prof) {
+      for (int k = 0; k < 4; ++k) atomicAdd(prof + k, ph[k]);
+      atomicAdd(prof + 4, steps);
+    }
   }
+#undef MM_PHASE
 }
 
 __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, double* s2all, double* distall,
@@ -713,6 +800,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   const size_t select_lds =
       (size_t)sortcap * sizeof(double) + (size_t)((sortcap + 1) / 2) * sizeof(double);
   MMState* hst = static_cast<MMState*>(ctx->stage.get(nd * sizeof(MMState)));
+  unsigned long long* mprof = nullptr;   // per-phase clocks of mm_select (profiling runs)
+  if (std::getenv("SVH_MM_PROFILE")) {
+    mprof = ctx->arena.get_t<unsigned long long>("mm.prof", 8);
+    const unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    SVH_HIP(hipMemcpy(mprof, z, sizeof z, hipMemcpyHostToDevice));
+  }
   int batch = 8;
   hipEvent_t mm_t0 = nullptr, mm_t1 = nullptr;
   if (ctx->timers.enabled) {
@@ -724,26 +817,27 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
     for (int r = 0; r < batch; ++r) {
       ctx->timers.begin(ctx->stream, "maximin_apply");
       const dim3 ga(maxslice, nd);
+      const dim3 ba(kSlice * kApplyParts);
       if (dm == 4)
-        hipLaunchKernelGGL(mm_apply_kernel<4>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+        hipLaunchKernelGGL(mm_apply_kernel<4>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
                            acc, st, ck, ci);
       else if (dm == 8)
-        hipLaunchKernelGGL(mm_apply_kernel<8>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+        hipLaunchKernelGGL(mm_apply_kernel<8>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
                            acc, st, ck, ci);
       else
-        hipLaunchKernelGGL(mm_apply_kernel<12>, ga, dim3(kSlice), 0, ctx->stream, dd, dXT, s2, dist,
+        hipLaunchKernelGGL(mm_apply_kernel<12>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
                            acc, st, ck, ci);
       ctx->timers.end(ctx->stream);
       ctx->timers.begin(ctx->stream, "maximin_select");
       if (dm == 4)
         hipLaunchKernelGGL(mm_select_kernel<4>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
       else if (dm == 8)
         hipLaunchKernelGGL(mm_select_kernel<8>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
       else
         hipLaunchKernelGGL(mm_select_kernel<12>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap);
+                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
       ctx->timers.end(ctx->stream);
     }
     SVH_HIP(hipGetLastError());
@@ -770,6 +864,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int rounds = 0;
   for (int i = 0; i < nd; ++i) rounds = std::max(rounds, hst[i].rounds);
   ctx->mm_rounds = rounds;
+  if (mprof) {
+    unsigned long long h[8];
+    SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));
+    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu sort %llu window %llu "
+                 "update %llu\n", rounds, h[4], h[0], h[1], h[2], h[3]);
+  }
 }
 
 void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t first,

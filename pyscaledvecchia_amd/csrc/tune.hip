@@ -9,8 +9,9 @@
 #include "svh_internal.h"
 
 #include <algorithm>
-#include <cstring>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 
@@ -67,6 +68,8 @@ int tune_choose(svh_ctx* ctx, const std::string& key, int ncand, int dflt,
       t[r] = ms;
     }
     std::sort(t, t + 3);
+    if (std::getenv("SVH_TUNE_LOG"))
+      std::fprintf(stderr, "[svh tune] %s candidate %d: %.4f ms\n", key.c_str(), i, t[1]);
     if (t[1] < best_ms) {
       best_ms = t[1];
       best = i;
