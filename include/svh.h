@@ -97,6 +97,23 @@ int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const do
  *   untouched when need_grad == 0.  *jitter_power_max receives the largest
  *   jitter power used by any chunk (0 = none, else -12..-4).
  * Replaces likelihood.py:66-175 (vecchia_profile_loglik) for fixed nu. */
+/* The whole rebuild of a fit at new ranges, on the device: the estimation
+ * data Xe (n x d), ye (n), Ze (n x q, NULL when q == 0) -- uploaded when
+ * upload_data is nonzero, otherwise the copy of the previous call with the
+ * same n, d, q is reused -- scaled by lam (d positive ranges, X / lam as the
+ * host computes it), the maximin order from `first` (svh_maximin), the
+ * ordered m nearest neighbours of the reordered scaled rows (svh_ordered_knn,
+ * start 0), and the structure svh_vecchia_setup would receive for the groups
+ * of ordering.py:119-127: blocks in row order, block i conditioning row i on
+ * its min(i, m) neighbours, jitter chunks of _covariance.py:255-259.  Writes
+ * the order (n) to `order`; the neighbour lists stay on the device.
+ * Requires 0 <= m < n, m <= 126, d <= 12.  One call replaces
+ * gp.py:182-191 (maximin_order, find_ordered_nn, _nn_groups and the
+ * structure upload) without moving the neighbour lists through the host. */
+int svh_vecchia_rebuild(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
+                        int64_t n, int32_t d, int32_t q, const double* lam, int64_t first,
+                        int32_t m, int32_t upload_data, int64_t* order);
+
 int svh_vecchia_eval(svh_ctx* ctx, const double* theta, int32_t P, double nu,
                      int32_t precision, int32_t need_grad, double var_penalty,
                      double log_var_target, double* ll, double* grad, double* fisher,

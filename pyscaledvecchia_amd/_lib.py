@@ -52,6 +52,9 @@ def _load():
     L.svh_vecchia_setup.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32,
                                     ctypes.c_int32, ctypes.c_int64, _i64p, _i64p, _i32p,
                                     ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, ctypes.c_int32]
+    L.svh_vecchia_rebuild.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.c_int32, _dp, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.c_int32, _i64p]
     L.svh_vecchia_eval.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                    _dp, _dp, _i32p]
@@ -98,7 +101,7 @@ def _load():
     for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
                  "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
-                 "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
+                 "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -112,7 +115,7 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_batch_eval", "svh_ordering_batch", "svh_mlp_init", "svh_mlp_num_params",
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
-            "svh_autotune_read")
+            "svh_autotune_read", "svh_vecchia_rebuild")
 
 
 def device_count() -> int:
@@ -258,6 +261,26 @@ class Context:
             _ptr(rows, _i64p), _ptr(nbr, _i64p), _ptr(kcnt, _i32p), int(mmax), _ptr(cg, _i32p),
             _ptr(cv, _i32p), int(nchunk_grad), int(nchunk_val)))
         self._q = q
+
+    def vecchia_rebuild(self, Xe, ye, Ze, lam, first, m, upload=True):
+        """Device rebuild at ranges `lam` (svh_vecchia_rebuild); returns the
+        maximin order.  With upload=False the data of the previous call are
+        reused."""
+        lam = _f64(lam)
+        n, d = (Xe.shape if Xe is not None else (self._rb_n, self._rb_d))
+        if upload:
+            Xe, ye = _finite(Xe, "Xe"), _finite(ye, "ye")
+            q = 0 if Ze is None else Ze.shape[1]
+            Ze = None if q == 0 else _f64(Ze)
+            self._rb_n, self._rb_d, self._rb_q = n, d, q
+        q = self._rb_q
+        order = np.empty(n, dtype=np.int64)
+        self._check(self._L.svh_vecchia_rebuild(
+            self._h, _ptr(Xe) if upload else None, _ptr(ye) if upload else None,
+            _ptr(Ze) if (upload and q) else None, n, d, q, _ptr(lam), int(first), int(m),
+            1 if upload else 0, _ptr(order, _i64p)))
+        self._q = q
+        return order
 
     def vecchia_eval(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
                      precision="fp64"):

@@ -7,6 +7,8 @@
 
 #include "svh_internal.h"
 
+#include <cmath>
+
 void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const double* dlam,
                    double* dXT);
 void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, const double* Zo,
@@ -188,6 +190,17 @@ int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const do
     SVH_REQUIRE(mmax == 0 || nbr, "setup: nbr is NULL");
     vecchia_setup_device(ctx, Xo, yo, Zo, n, d, q, nblk, rows, nbr, kcnt, mmax, chunk_grad,
                          chunk_val, nchunk_grad, nchunk_val);
+  });
+}
+
+int svh_vecchia_rebuild(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
+                        int64_t n, int32_t d, int32_t q, const double* lam, int64_t first,
+                        int32_t m, int32_t upload_data, int64_t* order) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(lam && order, "rebuild: null pointer");
+    for (int l = 0; l < d && l < 32; ++l)
+      SVH_REQUIRE(std::isfinite(lam[l]) && lam[l] > 0.0, "rebuild: ranges must be positive");
+    vecchia_rebuild_device(ctx, Xe, ye, Ze, n, d, q, lam, first, m, upload_data, order);
   });
 }
 

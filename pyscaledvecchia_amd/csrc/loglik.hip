@@ -1801,6 +1801,111 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
   vs.ready = true;
 }
 
+// ---------------------------------------------------------------- device rebuild
+// Row i of the rebuilt structure: inputs, response and design of point
+// order[i]; block i conditions row i on nn[i, 1..min(i, m)] (the groups of
+// k = min(i, m) are consecutive rows, so blocks run in row order); the jitter
+// chunk of block i is i below m (one-row groups) and m + (i - m) / step in the
+// k = m group (steps for the gradient and value passes).
+__global__ void gather_soa(const double* __restrict__ XT, const int64_t* __restrict__ order,
+                           int64_t n, int d, double* __restrict__ XTo) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * d) return;
+  const int64_t l = t / n, i = t - l * n;
+  XTo[t] = XT[l * n + order[i]];
+}
+
+__global__ void rebuild_rows(const double* __restrict__ Xe, const double* __restrict__ ye,
+                             const double* __restrict__ Ze, const int64_t* __restrict__ order,
+                             const int64_t* __restrict__ nn, int64_t n, int d, int q, int m,
+                             int mcol, int64_t stepg, int64_t stepv, double* __restrict__ X,
+                             double* __restrict__ y, double* __restrict__ Z, int* __restrict__ rows,
+                             int* __restrict__ nbr, int* __restrict__ kcnt, int* __restrict__ cg,
+                             int* __restrict__ cv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t src = order[i];
+  for (int l = 0; l < d; ++l) X[i * d + l] = Xe[src * d + l];
+  y[i] = ye[src];
+  for (int c = 0; c < q; ++c) Z[i * q + c] = Ze[src * q + c];
+  const int k = (int)(i < m ? i : m);
+  rows[i] = (int)i;
+  kcnt[i] = k;
+  for (int t = 0; t < mcol; ++t) nbr[i * mcol + t] = t < k ? (int)nn[i * (m + 1) + 1 + t] : -1;
+  cg[i] = (int)(i < m ? i : m + (i - m) / stepg);
+  cv[i] = (int)(i < m ? i : m + (i - m) / stepv);
+}
+
+void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
+                            int64_t n, int d, int q, const double* lam, int64_t first, int m,
+                            int upload_data, int64_t* order) {
+  VecchiaState& vs = ctx->vs;
+  vs.ready = false;
+  SVH_REQUIRE(n > 0 && d > 0 && d <= 32 && q >= 0 && q <= 32, "rebuild: bad n/d/q");
+  SVH_REQUIRE(n < (1ll << 29), "rebuild: n too large");
+  SVH_REQUIRE(m >= 0 && m < n && m + 1 <= kMaxKP, "rebuild: need 0 <= m < n, m <= 126");
+  SVH_REQUIRE(first >= 0 && first < n, "rebuild: first out of range");
+  hipStream_t s = ctx->stream;
+  // unordered estimation data stay resident between rebuilds of one fit
+  double* dXe = ctx->arena.get_t<double>("rb.X", (size_t)n * d);
+  double* dye = ctx->arena.get_t<double>("rb.y", n);
+  double* dZe = ctx->arena.get_t<double>("rb.Z", (size_t)n * (q > 0 ? q : 1));
+  if (upload_data) {
+    SVH_REQUIRE(Xe && ye && (q == 0 || Ze), "rebuild: null data pointer");
+    SVH_HIP(hipMemcpyAsync(dXe, Xe, (size_t)n * d * sizeof(double), hipMemcpyHostToDevice, s));
+    SVH_HIP(hipMemcpyAsync(dye, ye, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+    if (q > 0)
+      SVH_HIP(hipMemcpyAsync(dZe, Ze, (size_t)n * q * sizeof(double), hipMemcpyHostToDevice, s));
+    ctx->rb_n = n;
+    ctx->rb_d = d;
+    ctx->rb_q = q;
+  }
+  SVH_REQUIRE(ctx->rb_n == n && ctx->rb_d == d && ctx->rb_q == q,
+              "rebuild: data not uploaded for these sizes");
+  double* dlam = ctx->arena.get_t<double>("rb.lam", 32);
+  double* dXT = ctx->arena.get_t<double>("rb.XT", (size_t)n * d);
+  double* dXTo = ctx->arena.get_t<double>("rb.XTo", (size_t)n * d);
+  int64_t* dord = ctx->arena.get_t<int64_t>("rb.order", n);
+  int64_t* dnn = ctx->arena.get_t<int64_t>("rb.nn", (size_t)n * (m + 1));
+  SVH_HIP(hipMemcpyAsync(dlam, lam, d * sizeof(double), hipMemcpyHostToDevice, s));
+  // X / lambda in SoA (the same correctly rounded quotients as the host's)
+  launch_to_soa(ctx, dXe, n, d, dlam, dXT);
+  launch_maximin(ctx, dXT, n, d, first, dord);
+  const int64_t tot = n * d;
+  hipLaunchKernelGGL(gather_soa, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dXT, dord,
+                     n, d, dXTo);
+  SVH_HIP(hipGetLastError());
+  launch_ordered_knn(ctx, dXTo, n, d, m, 0, dnn);
+  const int P = d + 2, K = m + 1, mcol = m > 0 ? m : 1;
+  const int64_t stepg = std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K * P));
+  const int64_t stepv = std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K));
+  double* vX = ctx->arena.get_t<double>("v.X", (size_t)n * d);
+  double* vy = ctx->arena.get_t<double>("v.y", n);
+  double* vZ = ctx->arena.get_t<double>("v.Z", (size_t)n * (q > 0 ? q : 1));
+  int* vrows = ctx->arena.get_t<int>("v.rows", n);
+  int* vnbr = ctx->arena.get_t<int>("v.nbr", (size_t)n * mcol);
+  int* vk = ctx->arena.get_t<int>("v.kcnt", n);
+  int* vcg = ctx->arena.get_t<int>("v.cg", n);
+  int* vcv = ctx->arena.get_t<int>("v.cv", n);
+  hipLaunchKernelGGL(rebuild_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dXe, dye,
+                     dZe, dord, dnn, n, d, q, m, mcol, stepg, stepv, vX, vy, vZ, vrows, vnbr, vk,
+                     vcg, vcv);
+  SVH_HIP(hipGetLastError());
+  SVH_HIP(hipMemcpyAsync(order, dord, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  ctx->timers.collect();
+  const int64_t rest = n - m;   // rows of the k = m group (n > m)
+  vs.n = n;
+  vs.d = d;
+  vs.q = q;
+  vs.nblk = n;
+  vs.mmax = m;
+  vs.kp = (m + 1 <= 32) ? 32 : (m + 1 <= 64 ? 64 : 128);
+  vs.nchunk_grad = (int)(m + (rest + stepg - 1) / stepg);
+  vs.nchunk_val = (int)(m + (rest + stepv - 1) / stepv);
+  vs.ready = true;
+}
+
 void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, int precision,
                          int need_grad, double var_penalty, double log_var_target, double* ll,
                          double* grad, double* fisher, double* beta, int* jitter_power_max) {

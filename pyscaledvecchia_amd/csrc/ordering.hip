@@ -721,34 +721,223 @@ __global__ __launch_bounds__(kWave) void knn_kernel(const KNNDesc* __restrict__ 
   }
 }
 
+// Tiled form: kKnnWaves queries per workgroup (one wave each) share every
+// candidate tile, staged once in LDS in SoA layout; a lane takes one candidate
+// of a 64-wide chunk as above.  The list + buffer merge is a bitonic sort in
+// registers (LCAP / 64 entries per lane, lane exchanges by DPP / swizzle), so
+// the waves of a workgroup never wait on each other outside the tile loads.
+// Query coordinates sit in DM >= d registers (extra coordinates zero: each
+// adds 0*0 = +0 to the canonical sum).
+constexpr int kKnnWaves = 8;
+constexpr int kKnnTile = 256;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One ascending (key, idx) bitonic compare-exchange stage over E registers per
+// lane; position of register e in lane l is e * 64 + l.
+template <int E, int SIZE, int STRIDE>
+__device__ __forceinline__ void knn_sort_stage(double (&kk)[E], int (&ii)[E]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if constexpr (STRIDE >= kWave) {
+    constexpr int R = STRIDE / kWave;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((e & R) == 0) {
+        const int f = e | R;
+        const bool asc = ((e * kWave) & SIZE) == 0;
+        const bool sw = asc ? nn_less(kk[f], ii[f], kk[e], ii[e]) : nn_less(kk[e], ii[e], kk[f], ii[f]);
+        const double ka = kk[e];
+        const int ia = ii[e];
+        kk[e] = sw ? kk[f] : kk[e];
+        ii[e] = sw ? ii[f] : ii[e];
+        kk[f] = sw ? ka : kk[f];
+        ii[f] = sw ? ia : ii[f];
+      }
+    }
+  } else {
+    const bool lo = (lane & STRIDE) == 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double k2 = xchg<STRIDE>(kk[e]);
+      const int i2 = xchg<STRIDE>(ii[e]);
+      const bool asc = ((e * kWave + lane) & SIZE) == 0;
+      const bool eq = k2 == kk[e];
+      const bool pless = (k2 < kk[e]) | (eq & (i2 < ii[e]));
+      const bool pmore = (k2 > kk[e]) | (eq & (i2 > ii[e]));
+      const bool take = (asc == lo) ? pless : pmore;
+      kk[e] = take ? k2 : kk[e];
+      ii[e] = take ? i2 : ii[e];
+    }
+  }
+  if constexpr (STRIDE > 1) knn_sort_stage<E, SIZE, STRIDE / 2>(kk, ii);
+}
+
+template <int E, int SIZE>
+__device__ __forceinline__ void knn_sort(double (&kk)[E], int (&ii)[E]) {
+  knn_sort_stage<E, SIZE, SIZE / 2>(kk, ii);
+  if constexpr (SIZE < E * kWave) knn_sort<E, SIZE * 2>(kk, ii);
+}
+
+template <int DM, int LCAP>
+__global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
+    const KNNDesc* __restrict__ descs) {
+  constexpr int E = LCAP / kWave;
+  const KNNDesc D = descs[blockIdx.y];
+  const double* __restrict__ XT = D.XT;
+  const int64_t nref = D.nref, rowstart = D.rowstart;
+  const int d = D.d, m = D.m, ordered = D.ordered, out_stride = D.out_stride;
+  const int64_t nqry = ordered ? nref : D.nq;
+  const int64_t q0 = (int64_t)blockIdx.x * kKnnWaves;
+  if (q0 >= nqry) return;
+  __shared__ double tile[DM * kKnnTile];
+  __shared__ double lkey[kKnnWaves][LCAP];
+  __shared__ int lidx[kKnnWaves][LCAP];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int64_t q = q0 + w;
+  const bool qvalid = q < nqry;
+  int64_t hi = 0;
+  int k = 0;
+  int64_t* orow = D.out + q * (int64_t)out_stride;
+  if (qvalid) {
+    if (ordered) {
+      hi = q;
+      k = (int)((q < (int64_t)m) ? q : (int64_t)m);
+      if (q < rowstart) k = 0;
+      for (int t = lane; t <= m; t += kWave) orow[t] = (t == 0) ? q : -1;
+    } else {
+      hi = nref;
+      k = m;
+    }
+  }
+  // candidates of the workgroup: up to the last query's bound
+  const int64_t qlast = (q0 + kKnnWaves - 1 < nqry ? q0 + kKnnWaves - 1 : nqry - 1);
+  const int64_t hwg = ordered ? qlast : nref;
+  double qx[DM];
+  const double* qcol = ordered ? XT : D.QT;
+#pragma unroll
+  for (int l = 0; l < DM; ++l) qx[l] = (qvalid && l < d) ? qcol[(int64_t)l * nqry + q] : 0.0;
+  const double kInf = __longlong_as_double(0x7ff0000000000000ll);
+  double* key = lkey[w];
+  int* idx = lidx[w];
+  for (int t = lane; t < LCAP; t += kWave) {
+    key[t] = kInf;
+    idx[t] = INT_MAX;
+  }
+  double thr = kInf;
+  int thr_i = INT_MAX;
+  int nbuf = 0;
+  const int cap = LCAP - k;
+
+  auto merge = [&]() {
+    wave_sync();
+    double kk[E];
+    int ii[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      kk[e] = key[e * kWave + lane];
+      ii[e] = idx[e * kWave + lane];
+    }
+    knn_sort<E, 2>(kk, ii);
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int p = e * kWave + lane;
+      key[p] = p < k ? kk[e] : kInf;
+      idx[p] = p < k ? ii[e] : INT_MAX;
+    }
+    wave_sync();
+    thr = key[k - 1];
+    thr_i = idx[k - 1];
+    nbuf = 0;
+  };
+
+  for (int64_t base = 0; base < hwg; base += kKnnTile) {
+    __syncthreads();   // the previous tile is consumed
+    for (int e = threadIdx.x; e < DM * kKnnTile; e += kWave * kKnnWaves) {
+      const int l = e / kKnnTile, t = e - l * kKnnTile;
+      const int64_t j = base + t;
+      tile[e] = (l < d && j < nref) ? XT[(int64_t)l * nref + j] : 0.0;
+    }
+    __syncthreads();
+    if (k <= 0) continue;
+    const int64_t rem = hi - base;
+    const int tend = rem < kKnnTile ? (int)rem : kKnnTile;
+    for (int c = 0; c < tend; c += kWave) {
+      const int t = c + lane;
+      const int j = (int)(base + t);
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < DM; ++l) s = sq_step(s, qx[l], tile[l * kKnnTile + t]);
+      bool pass = t < tend && nn_less(s, j, thr, thr_i);
+      unsigned long long msk = __ballot(pass);
+      int cnt = __popcll(msk);
+      if (cnt == 0) continue;
+      if (nbuf + cnt > cap) {
+        merge();
+        pass = t < tend && nn_less(s, j, thr, thr_i);
+        msk = __ballot(pass);
+        cnt = __popcll(msk);
+      }
+      if (pass) {
+        const int pos = k + nbuf + __popcll(msk & ((1ull << lane) - 1ull));
+        key[pos] = s;
+        idx[pos] = j;
+      }
+      nbuf += cnt;
+    }
+  }
+  if (k <= 0) return;
+  merge();
+  wave_sync();
+  if (ordered) {
+    for (int t = lane; t < k; t += kWave) orow[1 + t] = idx[t];
+  } else {
+    for (int t = lane; t < k; t += kWave) orow[t] = idx[t];
+  }
+}
+
 template <int LCAP>
-void launch_knn_t(svh_ctx* ctx, const std::vector<KNNDesc>& h, int64_t maxq) {
+void launch_knn_t(svh_ctx* ctx, const std::vector<KNNDesc>& h, int64_t maxq, int dmax) {
   if (maxq == 0 || h.empty()) return;
   KNNDesc* dd = ctx->arena.get_t<KNNDesc>("knn.desc", h.size());
   SVH_HIP(hipMemcpyAsync(dd, h.data(), h.size() * sizeof(KNNDesc), hipMemcpyHostToDevice,
                          ctx->stream));
   ctx->timers.begin(ctx->stream, "knn");
-  hipLaunchKernelGGL((knn_kernel<LCAP>), dim3((unsigned)maxq, (unsigned)h.size()), dim3(kWave), 0,
-                     ctx->stream, dd);
+  const dim3 gt((unsigned)((maxq + kKnnWaves - 1) / kKnnWaves), (unsigned)h.size());
+  const dim3 bt(kWave * kKnnWaves);
+  if (dmax <= 4)
+    hipLaunchKernelGGL((knn_tiled_kernel<4, LCAP>), gt, bt, 0, ctx->stream, dd);
+  else if (dmax <= 8)
+    hipLaunchKernelGGL((knn_tiled_kernel<8, LCAP>), gt, bt, 0, ctx->stream, dd);
+  else if (dmax <= kMMD)
+    hipLaunchKernelGGL((knn_tiled_kernel<kMMD, LCAP>), gt, bt, 0, ctx->stream, dd);
+  else
+    hipLaunchKernelGGL((knn_kernel<LCAP>), dim3((unsigned)maxq, (unsigned)h.size()), dim3(kWave), 0,
+                       ctx->stream, dd);
   SVH_HIP(hipGetLastError());
   ctx->timers.end(ctx->stream);
 }
 
 void launch_knn_descs(svh_ctx* ctx, const std::vector<KNNDesc>& h) {
-  int mmax = 0;
+  int mmax = 0, dmax = 0;
   int64_t maxq = 0;
   for (const auto& D : h) {
     SVH_REQUIRE(D.m <= 448, "k/m above 448 not supported by the kNN kernel");
     mmax = std::max(mmax, D.m);
+    dmax = std::max(dmax, D.d);
     maxq = std::max<int64_t>(maxq, D.ordered ? D.nref : D.nq);
   }
   SVH_REQUIRE(maxq < (1ll << 31), "kNN: too many queries");
   if (mmax <= 64)
-    launch_knn_t<128>(ctx, h, maxq);
+    launch_knn_t<128>(ctx, h, maxq, dmax);
   else if (mmax <= 192)
-    launch_knn_t<256>(ctx, h, maxq);
+    launch_knn_t<256>(ctx, h, maxq, dmax);
   else
-    launch_knn_t<512>(ctx, h, maxq);
+    launch_knn_t<512>(ctx, h, maxq, dmax);
 }
 
 }  // namespace

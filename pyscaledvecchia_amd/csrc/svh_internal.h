@@ -173,6 +173,8 @@ struct svh_ctx {
   VecchiaState vs;
   KernelTimers timers;
   int mm_rounds = 0;  // rounds of the last maximin call
+  int64_t rb_n = 0;   // sizes of the estimation data resident for rebuilds
+  int rb_d = 0, rb_q = 0;
   MlpState* mlp = nullptr;
   Autotune tune;
   std::string last_error;
@@ -186,7 +188,12 @@ void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m
                         int64_t start, int64_t* dnn);
 void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const double* dXqT,
                       int64_t nq, int d, int k, int64_t* dnb, int nb_stride);
+void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const double* dlam,
+                   double* dXT);
 // loglik.hip
+void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
+                            int64_t n, int d, int q, const double* lam, int64_t first, int m,
+                            int upload_data, int64_t* order);
 void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, int precision,
                          int need_grad, double var_penalty, double log_var_target,
                          double* ll, double* grad, double* fisher, double* beta,

@@ -23,7 +23,7 @@ from scipy import sparse
 from scipy.sparse.linalg import spsolve_triangular
 
 from . import _lib
-from .likelihood import VecchiaStructure
+from .likelihood import DeviceRebuild, VecchiaStructure
 from .optimize import _fisher_scoring
 from .ordering import _nn_groups, find_ordered_nn, maximin_order
 
@@ -224,12 +224,19 @@ class ScaledVecchiaGP:
         self.n_rebuilds_ = 0
         self.n_evals_ = 0
 
+        dev = DeviceRebuild(Xe, ye, Ze, m, ctx=ctx) if DeviceRebuild.supported(d, m) else None
+
         def rebuild(th):
-            order, nn = self._order_structure(Xe, th, m)
-            groups = _nn_groups(nn, m)
-            state["order"] = order
-            state["groups"] = groups
-            state["struct"] = VecchiaStructure(Xe[order], ye[order], Ze[order], groups, ctx=ctx)
+            if dev is not None:
+                # maximin order, ordered neighbours and structure on the device
+                state["order"] = dev(np.exp(np.asarray(th, float)[1:1 + d]))
+                state["struct"] = dev.structure
+            else:
+                order, nn = self._order_structure(Xe, th, m)
+                groups = _nn_groups(nn, m)
+                state["order"] = order
+                state["struct"] = VecchiaStructure(Xe[order], ye[order], Ze[order], groups,
+                                                   ctx=ctx)
             self.n_rebuilds_ += 1
 
         rebuild(theta)

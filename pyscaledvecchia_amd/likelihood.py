@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["vecchia_profile_loglik", "VecchiaStructure"]
+__all__ = ["vecchia_profile_loglik", "VecchiaStructure", "DeviceRebuild"]
 
 _CHUNK_BUDGET = 4_000_000     # _covariance.py:255
 
@@ -65,6 +65,41 @@ class VecchiaStructure:
 
     def __call__(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
                  precision="fp64"):
+        return self.ctx.vecchia_eval(theta, nu, need_grad=need_grad, var_penalty=var_penalty,
+                                     log_var_target=log_var_target, precision=precision)
+
+
+class DeviceRebuild:
+    """The rebuilds of one fit's Vecchia structure on the device
+    (svh_vecchia_rebuild): the estimation data are uploaded by the first call,
+    every call reorders them at new ranges (maximin order, ordered neighbours,
+    structure) without moving the neighbour lists through the host.  Calling
+    the object's `structure` evaluates like a VecchiaStructure."""
+
+    def __init__(self, Xe, ye, Ze, m, ctx=None):
+        self.ctx = ctx or _lib.context()
+        self.Xe = np.ascontiguousarray(Xe, dtype=np.float64)
+        self.ye = np.ascontiguousarray(ye, dtype=np.float64)
+        self.Ze = None if (Ze is None or Ze.shape[1] == 0) else np.ascontiguousarray(Ze, float)
+        self.m = int(m)
+
+    @staticmethod
+    def supported(d, m):
+        return d <= 12 and m + 1 <= 127
+
+    def __call__(self, lam):
+        """Reorder at ranges `lam`; returns the maximin order."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        Xs = self.Xe / lam
+        first = int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1)))   # ordering.py:72
+        fresh = getattr(self.ctx, "_rb_owner", None) is not self
+        order = self.ctx.vecchia_rebuild(self.Xe if fresh else None, self.ye, self.Ze, lam,
+                                         first, self.m, upload=fresh)
+        self.ctx._rb_owner = self
+        return order
+
+    def structure(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
+                  precision="fp64"):
         return self.ctx.vecchia_eval(theta, nu, need_grad=need_grad, var_penalty=var_penalty,
                                      log_var_target=log_var_target, precision=precision)
 
