@@ -74,28 +74,28 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 //             unselected point (s2 = min squared distance, dist = sqrt(s2));
 //             each 256-point slice publishes its top-(kSliceTop+1) points by
 //             (dist desc, index asc).
-//   mm_select (one workgroup, one pool slot per thread): V = the largest
-//             published "next" element over all slices is an upper bound (in
-//             that order) on every point that is not a published candidate.
-//             Candidates above V form a pool, kept sorted by (dist desc, index
-//             asc).  A step accepts the longest prefix c_1..c_L of the sorted
-//             pool (at most kWin entries) in which no c_i is lowered by an
-//             earlier c_j (squared distance below c_i's own): c_1 is the
-//             serial algorithm's next pick, and by induction so is each c_i --
-//             its distance is unchanged by c_1..c_{i-1}, every later pool entry
-//             was at or below it and distances only fall, and every point
-//             outside the pool is at or below V.  The step then lowers the
-//             remaining pool with the L accepted points (entries falling to V
-//             or below leave the pool) and re-sorts it.  The round ends when
-//             the pool is empty.  On well-spread inputs most steps accept the
-//             whole window, so the serial chain of single picks becomes a few
-//             wide steps per round.
+//   mm_select (one workgroup): V = the largest published "next" element over
+//             all slices is an upper bound (in that order) on every point that
+//             is not a published candidate.  Candidates above V form a pool,
+//             kept sorted by (dist desc, index asc).  A step accepts the
+//             longest prefix c_1..c_L of the sorted pool (at most kWin
+//             entries) in which no c_i is lowered by an earlier c_j (squared
+//             distance below c_i's own): c_1 is the serial algorithm's next
+//             pick, and by induction so is each c_i -- its distance is
+//             unchanged by c_1..c_{i-1}, every later pool entry was at or below
+//             it and distances only fall, and every point outside the pool is
+//             at or below V.  The step then lowers the rest of the pool with
+//             the L accepted points; entries falling to V or below leave the
+//             pool, the others keep their order (a stable compaction), and
+//             only when some entry was lowered but stays above V is the pool
+//             sorted again.  The round ends when the pool is empty.  On
+//             well-spread inputs most steps accept the whole window.
 // Coordinates are held in DM >= d registers per point with the extra
 // coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
 // the squared distances are bit-identical to the d-term sums.
 constexpr int kSlice = 256;
 constexpr int kSliceTop = 4;          // published candidates per slice
-constexpr int kPool = 256;            // pool slots = select threads
+constexpr int kPool = 256;            // pool positions = select threads
 constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
 constexpr int kMMD = 12;              // max d of the maximin kernels
@@ -115,24 +115,121 @@ struct MMDesc {
   int n, d, first, slice0, nslice, pad;
 };
 
+// Lane exchange with lane ^ M inside a wave, for a compile-time M: DPP quad
+// permutes for 1 and 2, swizzles within 32 lanes for 4..16, a backwards
+// permute for 32.
+template <int M>
+__device__ __forceinline__ int xchg(int v) {
+  if constexpr (M == 1) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  } else if constexpr (M == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  } else if constexpr (M < 32) {
+    return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);            // lane ^ M within 32
+  } else {
+    return __builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ M) << 2), v);
+  }
+}
+
+template <int M>
+__device__ __forceinline__ double xchg(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = xchg<M>((int)(b & 0xffffffffll)), hi = xchg<M>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// (key, idx) orders of the register sorts: largest key first (maximin) or
+// smallest first (neighbours); equal keys by the smaller index.
+struct MaxFirst {
+  static constexpr bool kDesc = true;
+};
+struct MinFirst {
+  static constexpr bool kDesc = false;
+};
+
+// Is (ka, ia) ahead of (kb, ib) in order Ord?  Branch-free.
+template <class Ord>
+__device__ __forceinline__ bool ahead(double ka, int ia, double kb, int ib) {
+  const bool eq = ka == kb;
+  if constexpr (Ord::kDesc)
+    return (ka > kb) | (eq & (ia < ib));
+  else
+    return (ka < kb) | (eq & (ia < ib));
+}
+
+// One bitonic stage over E entries per lane (position e * 64 + lane) into
+// order Ord; both partners evaluate the same strict comparison.
+template <class Ord, int E, int SIZE, int STRIDE>
+__device__ __forceinline__ void reg_sort_stage(double (&kk)[E], int (&ii)[E]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if constexpr (STRIDE >= kWave) {
+    constexpr int R = STRIDE / kWave;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if ((e & R) == 0) {
+        const int f = e | R;
+        const bool up = ((e * kWave) & SIZE) == 0;
+        const bool sw = up ? ahead<Ord>(kk[f], ii[f], kk[e], ii[e])
+                           : ahead<Ord>(kk[e], ii[e], kk[f], ii[f]);
+        const double ka = kk[e];
+        const int ia = ii[e];
+        kk[e] = sw ? kk[f] : kk[e];
+        ii[e] = sw ? ii[f] : ii[e];
+        kk[f] = sw ? ka : kk[f];
+        ii[f] = sw ? ia : ii[f];
+      }
+    }
+  } else {
+    const bool lo = (lane & STRIDE) == 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double k2 = xchg<STRIDE>(kk[e]);
+      const int i2 = xchg<STRIDE>(ii[e]);
+      const bool up = ((e * kWave + lane) & SIZE) == 0;
+      const bool take = (up == lo) ? ahead<Ord>(k2, i2, kk[e], ii[e])
+                                   : ahead<Ord>(kk[e], ii[e], k2, i2);
+      kk[e] = take ? k2 : kk[e];
+      ii[e] = take ? i2 : ii[e];
+    }
+  }
+  if constexpr (STRIDE > 1) reg_sort_stage<Ord, E, SIZE, STRIDE / 2>(kk, ii);
+}
+
+template <class Ord, int E, int SIZE>
+__device__ __forceinline__ void reg_sort_size(double (&kk)[E], int (&ii)[E]) {
+  reg_sort_stage<Ord, E, SIZE, SIZE / 2>(kk, ii);
+  if constexpr (SIZE < E * kWave) reg_sort_size<Ord, E, SIZE * 2>(kk, ii);
+}
+
+// Bitonic sort of the E * 64 entries of a wave (E per lane, in registers).
+template <class Ord, int E>
+__device__ __forceinline__ void reg_sort(double (&kk)[E], int (&ii)[E]) {
+  reg_sort_size<Ord, E, 2>(kk, ii);
+}
+
 // One workgroup per slice, kApplyParts threads per point: part p applies the
 // accepted points p, p + kApplyParts, ...; the parts' minima are combined in
-// LDS (a minimum does not depend on the order of its terms).
+// LDS (a minimum does not depend on the order of its terms).  The accepted
+// points' indices and coordinates (kMMD doubles each, written by mm_select /
+// mm_init) are staged in LDS once per workgroup.  The slice's list is
+// published by register sorts: each wave of part 0 sorts its 64 points, wave
+// 0 sorts the waves' first 32 and writes the first kSliceTop + 1.
 constexpr int kApplyParts = 4;
 
 template <int DM>
 __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     double* __restrict__ s2all, double* __restrict__ distall, const int* __restrict__ accall,
-    const MMState* __restrict__ stall, double* __restrict__ cand_keyall,
-    int* __restrict__ cand_idxall) {
-  constexpr int NT = kSlice * kApplyParts;
+    const double* __restrict__ accxall, const MMState* __restrict__ stall,
+    double* __restrict__ cand_keyall, int* __restrict__ cand_idxall) {
+  constexpr int PW = kSlice / kWave;   // waves of part 0
+  static_assert(kSliceTop + 1 <= 32, "published list exceeds half a wave");
   __shared__ double accX[kAccCap * DM];
   __shared__ int accI[kAccCap];
   __shared__ double pmin[kApplyParts - 1][kSlice];
   __shared__ int ppick[kApplyParts - 1][kSlice];
-  __shared__ double wk[kSlice / kWave];
-  __shared__ int wi[kSlice / kWave];
+  __shared__ double tk[PW * 32];
+  __shared__ int ti[PW * 32];
   const MMDesc D = desc[blockIdx.y];
   if ((int)blockIdx.x >= D.nslice) return;
   const int n = D.n, d = D.d;
@@ -144,16 +241,18 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
   if (st->count >= n) return;   // order complete: nothing left to select
   double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
   int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
+  const double* accx = accxall + (int64_t)blockIdx.y * kAccCap * kMMD;
   const int A = st->nacc;
-  const int tid = threadIdx.x, part = tid / kSlice, pt = tid - part * kSlice;
-  for (int t = tid; t < A; t += NT) accI[t] = acc[t];
-  __syncthreads();
-  for (int t = tid; t < A * DM; t += NT) {
-    const int a = t / DM, l = t - a * DM;
-    accX[t] = (l < d) ? XT[(int64_t)l * n + accI[a]] : 0.0;
+  // part is the same on a whole wave (kSlice is a multiple of the wave size)
+  const int tid = threadIdx.x, part = __builtin_amdgcn_readfirstlane(tid / kSlice);
+  const int pt = tid - part * kSlice;
+  const int i = blockIdx.x * kSlice + pt;
+  for (int t = tid; t < A; t += kSlice * kApplyParts) accI[t] = acc[t];
+  for (int t = tid; t < A * DM; t += kSlice * kApplyParts) {
+    const int a = t / DM;
+    accX[t] = accx[a * kMMD + (t - a * DM)];
   }
   __syncthreads();
-  const int i = blockIdx.x * kSlice + pt;
   double di = (i < n) ? dist[i] : -1.0;
   const bool active = i < n && di >= 0.0 && A > 0;
   double smin = INFINITY;
@@ -177,53 +276,44 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     }
   }
   __syncthreads();
-  if (part == 0 && active) {
+  const int lane = tid & (kWave - 1), wid = tid / kWave;
+  if (part == 0) {
+    if (active) {
 #pragma unroll
-    for (int p = 0; p < kApplyParts - 1; ++p) {
-      const double v = pmin[p][pt];
-      smin = v < smin ? v : smin;
-      picked = picked || ppick[p][pt] != 0;
+      for (int p = 0; p < kApplyParts - 1; ++p) {
+        const double v = pmin[p][pt];
+        smin = v < smin ? v : smin;
+        picked = picked || ppick[p][pt] != 0;
+      }
+      if (picked) {
+        di = -1.0;
+        dist[i] = -1.0;
+      } else if (smin < s2[i]) {
+        s2[i] = smin;
+        di = sqrt(smin);
+        dist[i] = di;
+      }
     }
-    if (picked) {
-      di = -1.0;
-      dist[i] = -1.0;
-    } else if (smin < s2[i]) {
-      s2[i] = smin;
-      di = sqrt(smin);
-      dist[i] = di;
+    // this wave's 64 points sorted by (dist desc, index asc); its first 32
+    // go to LDS
+    double kk[1] = {(i < n) ? di : -1.0};
+    int ii[1] = {(i < n) ? i : INT_MAX};
+    reg_sort<MaxFirst, 1>(kk, ii);
+    if (lane < 32) {
+      tk[wid * 32 + lane] = kk[0];
+      ti[wid * 32 + lane] = ii[0];
     }
   }
-  // top-(kSliceTop+1) of this slice by (dist desc, index asc), on part 0
-  double k = (i < n) ? di : -1.0;
-  int ki = (i < n) ? i : INT_MAX;
-  const int lane = tid & (kWave - 1), wid = tid / kWave;
-  for (int r = 0; r <= kSliceTop; ++r) {
-    if (part == 0) {   // waves 0 .. kSlice/kWave - 1
-      double bk = k;
-      int bi = ki;
-      wave_argmax(bk, bi);
-      if (lane == 0) {
-        wk[wid] = bk;
-        wi[wid] = bi;
-      }
+  __syncthreads();
+  if (wid == 0) {
+    // the slice's top-(kSliceTop+1) from the waves' first 32
+    double kk[2] = {tk[lane], tk[kWave + lane]};
+    int ii[2] = {ti[lane], ti[kWave + lane]};
+    reg_sort<MaxFirst, 2>(kk, ii);
+    if (lane <= kSliceTop) {
+      cand_key[blockIdx.x * (kSliceTop + 1) + lane] = kk[0];
+      cand_idx[blockIdx.x * (kSliceTop + 1) + lane] = ii[0];
     }
-    __syncthreads();
-    double gk = wk[0];
-    int gi = wi[0];
-    for (int w = 1; w < kSlice / kWave; ++w)
-      if (mm_greater(wk[w], wi[w], gk, gi)) {
-        gk = wk[w];
-        gi = wi[w];
-      }
-    if (tid == 0) {
-      cand_key[blockIdx.x * (kSliceTop + 1) + r] = gk;
-      cand_idx[blockIdx.x * (kSliceTop + 1) + r] = gi;
-    }
-    if (ki == gi) {
-      k = -2.0;
-      ki = INT_MAX;
-    }
-    __syncthreads();
   }
 }
 
@@ -249,29 +339,6 @@ __device__ __forceinline__ void block_argmax(double& k, int& i, double* rk, int*
     }
 }
 
-// Lane exchange with lane ^ M inside a wave, for a compile-time M: DPP quad
-// permutes for 1 and 2, swizzles within 32 lanes for 4..16, a backwards
-// permute for 32.
-template <int M>
-__device__ __forceinline__ int xchg(int v) {
-  if constexpr (M == 1) {
-    return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-  } else if constexpr (M == 2) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
-  } else if constexpr (M < 32) {
-    return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);            // lane ^ M within 32
-  } else {
-    return __builtin_amdgcn_ds_bpermute((int)((__lane_id() ^ M) << 2), v);
-  }
-}
-
-template <int M>
-__device__ __forceinline__ double xchg(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = xchg<M>((int)(b & 0xffffffffll)), hi = xchg<M>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // One bitonic compare-exchange between position tid and tid ^ STRIDE inside
 // blocks of SIZE, (key desc, idx asc) order in the final block.  Both partners
 // evaluate the same strict comparison without branches, so a pair equal in
@@ -281,10 +348,7 @@ __device__ __forceinline__ void bitonic_step(double& k, int& i, int& s, double k
                                              int s2) {
   const int tid = threadIdx.x;
   const bool desc_ = (tid & SIZE) == 0, lo = (tid & STRIDE) == 0;
-  const bool eq = k2 == k;
-  const bool pg = (k2 > k) | (eq & (i2 < i));   // partner ahead in the order
-  const bool pl = (k2 < k) | (eq & (i2 > i));   // partner behind
-  const bool take = (desc_ == lo) ? pg : pl;
+  const bool take = (desc_ == lo) ? ahead<MaxFirst>(k2, i2, k, i) : ahead<MaxFirst>(k, i, k2, i2);
   k = take ? k2 : k;
   i = take ? i2 : i;
   s = take ? s2 : s;
@@ -328,10 +392,12 @@ template <int DM>
 __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     const double* __restrict__ s2all, const double* __restrict__ cand_keyall,
-    const int* __restrict__ cand_idxall, int* __restrict__ accall, MMState* __restrict__ stall,
-    int64_t* __restrict__ orderall, int sortcap, unsigned long long* __restrict__ prof) {
+    const int* __restrict__ cand_idxall, int* __restrict__ accall, double* __restrict__ accxall,
+    MMState* __restrict__ stall, int64_t* __restrict__ orderall, int sortcap,
+    unsigned long long* __restrict__ prof) {
   // prof (profiling runs only): shader clocks of thread 0 per phase -- set-up,
-  // sorts, window checks, pool updates -- and the number of steps
+  // window checks, pool updates and compactions, re-sorts -- and the number of
+  // steps
   unsigned long long ph[4] = {0, 0, 0, 0}, steps = 0;
   unsigned long long ph_t = prof ? __builtin_amdgcn_s_memtime() : 0;
 #define MM_PHASE(k)                                            \
@@ -341,6 +407,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     ph_t = t_;                                                 \
   }
   constexpr int NT = kPool, NW = NT / kWave;
+  static_assert(kWin == kWave && 4 * kWin <= NT, "window layout");
   const MMDesc D = desc[blockIdx.x];
   const int n = D.n, d = D.d, nslice = D.nslice;
   const double* XT = XTall + D.xoff;
@@ -348,54 +415,56 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   const double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
   const int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
   int* acc = accall + (int64_t)blockIdx.x * kAccCap;
+  double* accx = accxall + (int64_t)blockIdx.x * kAccCap * kMMD;
   MMState* st = stall + blockIdx.x;
   int64_t* order = orderall + D.soff;
-  // LDS: survivors (sortcap keys + sortcap indices), pool coordinates,
-  // per-wave reduction slots (double-buffered over picks)
+  // LDS: survivors (sortcap keys + sortcap indices); pool coordinates and
+  // squared distances by slot; the sorted pool (key, index, slot by
+  // position) in two buffers; the window's coordinates
   extern __shared__ double smem[];
   double* sk = smem;
   int* si = reinterpret_cast<int*>(smem + sortcap);
-  __shared__ double pc[NT * DM];
+  __shared__ double pc[NT * DM], pss[NT];
+  __shared__ double qk[2][NT];
+  __shared__ int qi[2][NT], qs[2][NT];
   __shared__ double rk[NW];
   __shared__ int ri[NW], wcnt[NW];
-  // sorted pool: key, index and slot per position; squared distance per slot
-  __shared__ double qk[NT], pss[NT];
-  __shared__ int qi[NT], qs[NT];
-  __shared__ unsigned long long wlow[NW];
+  __shared__ unsigned long long wlow[NW], wlwr[NW];
   __shared__ int stepL;
-  __shared__ double wc[kWin * DM], ws2[kWin];   // window coordinates, squared distances
-  __shared__ int accf[NT];                      // slot accepted in this step
+  __shared__ double wc[kWin * DM], ws2[kWin];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
   if (count0 >= n) {
     if (tid == 0) st->nacc = 0;
     return;
   }
-  // bound V: the largest "next" element over all slices
+  // bound V: the largest "next" element (index kSliceTop) over all slices;
+  // every point outside the slices' published lists is at or below it
+  const int T = kSliceTop;
   double Vk = -3.0;
   int Vi = INT_MAX;
   for (int s = tid; s < nslice; s += NT) {
-    const double k = cand_key[s * (kSliceTop + 1) + kSliceTop];
-    const int i = cand_idx[s * (kSliceTop + 1) + kSliceTop];
+    const double k = cand_key[s * (kSliceTop + 1) + T];
+    const int i = cand_idx[s * (kSliceTop + 1) + T];
     if (mm_greater(k, i, Vk, Vi)) {
       Vk = k;
       Vi = i;
     }
   }
   block_argmax<NT>(Vk, Vi, rk, ri);
-  // survivors: published candidates strictly above V, compacted into LDS.
+  // survivors: first-T candidates strictly above V, compacted into LDS.
   // Survivors that do not fit raise the bound instead (they leave the pool).
   int cnt = 0;
   double Ok = -3.0;
   int Oi = INT_MAX;
-  const int ncand = nslice * kSliceTop;
+  const int ncand = nslice * T;
   for (int base = 0; base < ncand; base += NT) {
     const int c = base + tid;
     double k = -3.0;
     int i = INT_MAX;
     bool keep = false;
     if (c < ncand) {
-      const int s = c / kSliceTop, r = c - s * kSliceTop;
+      const int s = c / T, r = c - s * T;
       k = cand_key[s * (kSliceTop + 1) + r];
       i = cand_idx[s * (kSliceTop + 1) + r];
       keep = k >= 0.0 && mm_greater(k, i, Vk, Vi);
@@ -468,107 +537,178 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     if (tid == 0) st->overflow_sorts += 1;
     __syncthreads();
   }
-  // pool: one slot per thread, coordinates in registers and in LDS
-  double pv = -3.0, ps = 0.0, px[DM];
-  int pi = INT_MAX;
+  // pool: slot t holds survivor t
   if (tid < nsurv) {
-    pv = sk[tid];
-    pi = si[tid];
-    ps = s2g[pi];
-  }
+    const int pi = si[tid];
+    pss[tid] = s2g[pi];
 #pragma unroll
-  for (int l = 0; l < DM; ++l) {
-    px[l] = (l < d && tid < nsurv) ? XT[(int64_t)l * n + pi] : 0.0;
-    pc[tid * DM + l] = px[l];
+    for (int l = 0; l < DM; ++l) pc[tid * DM + l] = (l < d) ? XT[(int64_t)l * n + pi] : 0.0;
+  } else {
+    pss[tid] = 0.0;
+#pragma unroll
+    for (int l = 0; l < DM; ++l) pc[tid * DM + l] = 0.0;
   }
-  accf[tid] = 0;
-  int nacc = 0;
+  {
+    double k = tid < nsurv ? sk[tid] : -3.0;
+    int i = tid < nsurv ? si[tid] : INT_MAX, sl = tid;
+    pool_sort<NT>(k, i, sl, qk[1], qi[1], qs[1]);
+    qk[0][tid] = k;
+    qi[0][tid] = i;
+    qs[0][tid] = sl;
+  }
+  __syncthreads();
+  int cur = 0, nlive = nsurv, nacc = 0;
   const int room = n - count0;
   const int cap = room < kAccCap ? room : kAccCap;
-  static_assert(kWin * 4 == NT && kWin == kWave, "window layout");
   MM_PHASE(0)
-  while (nacc < cap) {
+  while (nacc < cap && nlive > 0) {
     ++steps;
-    // sort the pool by (dist desc, index asc); slots travel with their keys
-    pss[tid] = ps;
-    {
-      double k = pv;
-      int i = pi, sl = tid;
-      pool_sort<NT>(k, i, sl, qk, qi, qs);
-      qk[tid] = k;
-      qi[tid] = i;
-      qs[tid] = sl;
-    }
-    __syncthreads();
+    const double* ck = qk[cur];
+    const int* cidx = qi[cur];
+    const int* cs = qs[cur];
     // the window's coordinates and squared distances, contiguous by position
+    const int nw = nlive < kWin ? nlive : kWin;
     for (int e = tid; e < kWin * DM; e += NT) {
       const int w = e / DM;
-      wc[e] = pc[qs[w] * DM + (e - w * DM)];
+      wc[e] = w < nw ? pc[cs[w] * DM + (e - w * DM)] : 0.0;
     }
-    if (tid < kWin) ws2[tid] = pss[qs[tid]];
+    if (tid < kWin) ws2[tid] = tid < nw ? pss[cs[tid]] : 0.0;
     __syncthreads();
-    MM_PHASE(1)
     // window conflicts: is c_w lowered by an earlier c_j?  Four threads per
     // window entry (consecutive lanes), each taking every fourth j.
-    const int w = tid >> 2, wsub = tid & 3;
-    bool low = false;
-    if (w > 0 && qk[w] >= 0.0) {
-      const double s_w = ws2[w];
-      double xw[DM];
+    {
+      const int w = tid >> 2, wsub = tid & 3;
+      bool low = false;
+      if (w > 0 && w < nw) {
+        const double s_w = ws2[w];
+        double xw[DM];
 #pragma unroll
-      for (int l = 0; l < DM; ++l) xw[l] = wc[w * DM + l];
-      for (int j = wsub; j < w; j += 4) {
-        double t = 0.0;
+        for (int l = 0; l < DM; ++l) xw[l] = wc[w * DM + l];
+        for (int j = wsub; j < w; j += 4) {
+          double t = 0.0;
 #pragma unroll
-        for (int l = 0; l < DM; ++l) t = sq_step(t, xw[l], wc[j * DM + l]);
-        low = low || (t < s_w);
+          for (int l = 0; l < DM; ++l) t = sq_step(t, xw[l], wc[j * DM + l]);
+          low = low || (t < s_w);
+        }
       }
+      const unsigned long long lb = __ballot(low);
+      if (lane == 0) wlow[wid] = lb;
     }
-    const unsigned long long lb = __ballot(low);
-    if (lane == 0) wlow[wid] = lb;
     __syncthreads();
     if (wid == 0) {
-      // window entry `lane`: bad when lowered or out of the pool
+      // window entry `lane`: bad when lowered or outside the pool
       const unsigned nib = (unsigned)(wlow[lane >> 4] >> (4 * (lane & 15))) & 0xFu;
-      const bool bad = (lane > 0 && nib != 0u) || !(qk[lane] >= 0.0);
+      const bool bad = (lane > 0 && nib != 0u) || lane >= nw;
       const unsigned long long bb = __ballot(bad);
       int L = bb ? (int)__ffsll((long long)bb) - 1 : kWin;
       if (L > cap - nacc) L = cap - nacc;
       if (lane == 0) stepL = L;
     }
     __syncthreads();
-    const int L = stepL;
-    MM_PHASE(2)
-    if (L == 0) break;   // pool empty: every remaining entry is at or below V
+    const int L = stepL;   // >= 1: position 0 is in the pool
+    MM_PHASE(1)
     if (tid < L) {
-      acc[nacc + tid] = qi[tid];
-      order[count0 + nacc + tid] = qi[tid];
-      accf[qs[tid]] = 1;
+      const int idx = cidx[tid];
+      acc[nacc + tid] = idx;
+      order[count0 + nacc + tid] = idx;
+#pragma unroll
+      for (int l = 0; l < kMMD; ++l) accx[(nacc + tid) * kMMD + l] = l < DM ? wc[tid * DM + l] : 0.0;
     }
-    __syncthreads();
-    // lower the rest of the pool with the accepted points (window positions
-    // 0..L-1); entries that fall to V or below cannot be picked this round
-    if (accf[tid]) {
-      accf[tid] = 0;
-      pv = -3.0;
-    } else if (pv >= 0.0) {
-      double smin = ps;
-#pragma unroll 4
-      for (int j = 0; j < L; ++j) {
+    // lower the rest of the pool (position tid) with the accepted points.
+    // Lane j of every wave holds accepted point j; the loop broadcasts its
+    // coordinates with readlane (scalar operands) instead of LDS reads.
+    double ax[DM];
+#pragma unroll
+    for (int l = 0; l < DM; ++l) ax[l] = lane < L ? wc[lane * DM + l] : 0.0;
+    double key = -3.0;
+    int idx = INT_MAX, slot = 0;
+    bool lowered = false;
+    if (tid >= L && tid < nlive) {
+      slot = cs[tid];
+      key = ck[tid];
+      idx = cidx[tid];
+      const double s = pss[slot];
+      double x[DM];
+#pragma unroll
+      for (int l = 0; l < DM; ++l) x[l] = pc[slot * DM + l];
+      // four independent distance chains per iteration (one wave per SIMD:
+      // the FP64 add latency is hidden by instruction-level parallelism)
+      double smin = s;
+      int j = 0;
+      for (; j + 4 <= L; j += 4) {
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          const long long b = __double_as_longlong(ax[l]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j + u);
+            const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j + u);
+            t[u] = sq_step(t[u], x[l], __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) smin = t[u] < smin ? t[u] : smin;
+      }
+      for (; j < L; ++j) {
         double t = 0.0;
 #pragma unroll
-        for (int l = 0; l < DM; ++l) t = sq_step(t, px[l], wc[j * DM + l]);
+        for (int l = 0; l < DM; ++l) {
+          const long long b = __double_as_longlong(ax[l]);
+          const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
+          const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+          t = sq_step(t, x[l], __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+        }
         smin = t < smin ? t : smin;
       }
-      if (smin < ps) {
-        ps = smin;
-        pv = sqrt(ps);
-        if (!mm_greater(pv, pi, Vk, Vi)) pv = -3.0;
+      if (smin < s) {
+        pss[slot] = smin;
+        key = sqrt(smin);
+        if (mm_greater(key, idx, Vk, Vi))
+          lowered = true;
+        else
+          key = -3.0;   // at or below V: leaves the pool
       }
     }
+    const bool live = key >= 0.0;
+    // stable compaction of the remaining entries into the other buffer
+    const unsigned long long lm = __ballot(live), lw = __ballot(lowered);
+    if (lane == 0) {
+      wcnt[wid] = __popcll(lm);
+      wlwr[wid] = lw;
+    }
+    __syncthreads();
+    int off = 0, tot = 0;
+    bool anylow = false;
+#pragma unroll
+    for (int w2 = 0; w2 < NW; ++w2) {
+      off += (w2 < wid) ? wcnt[w2] : 0;
+      tot += wcnt[w2];
+      anylow = anylow || wlwr[w2] != 0ull;
+    }
+    if (live) {
+      const int np = off + __popcll(lm & ((1ull << lane) - 1ull));
+      qk[1 - cur][np] = key;
+      qi[1 - cur][np] = idx;
+      qs[1 - cur][np] = slot;
+    }
     nacc += L;
-    __syncthreads();   // wc / qs are rewritten by the next step
-    MM_PHASE(3)
+    nlive = tot;
+    cur = 1 - cur;
+    __syncthreads();
+    MM_PHASE(2)
+    if (anylow) {
+      // an entry moved down but stays in the pool: sort the pool again
+      double k = tid < nlive ? qk[cur][tid] : -3.0;
+      int i = tid < nlive ? qi[cur][tid] : INT_MAX, sl = tid < nlive ? qs[cur][tid] : 0;
+      pool_sort<NT>(k, i, sl, qk[1 - cur], qi[1 - cur], qs[1 - cur]);
+      __syncthreads();
+      qk[cur][tid] = k;
+      qi[cur][tid] = i;
+      qs[cur][tid] = sl;
+      __syncthreads();
+      MM_PHASE(3)
+    }
   }
   if (tid == 0) {
     st->nacc = nacc;
@@ -582,8 +722,9 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
 #undef MM_PHASE
 }
 
-__global__ void mm_init_kernel(const MMDesc* __restrict__ desc, double* s2all, double* distall,
-                               int* accall, MMState* stall, int64_t* orderall) {
+__global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
+                               double* s2all, double* distall, int* accall, double* accxall,
+                               MMState* stall, int64_t* orderall) {
   const MMDesc D = desc[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < D.n) {
@@ -592,6 +733,9 @@ __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, double* s2all, d
   }
   if (i == 0) {
     accall[(int64_t)blockIdx.y * kAccCap] = D.first;
+    for (int l = 0; l < kMMD; ++l)
+      accxall[(int64_t)blockIdx.y * kAccCap * kMMD + l] =
+          l < D.d ? XTall[D.xoff + (int64_t)l * D.n + D.first] : 0.0;
     orderall[D.soff] = D.first;
     MMState& st = stall[blockIdx.y];
     st.count = 1;
@@ -737,51 +881,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One ascending (key, idx) bitonic compare-exchange stage over E registers per
-// lane; position of register e in lane l is e * 64 + l.
-template <int E, int SIZE, int STRIDE>
-__device__ __forceinline__ void knn_sort_stage(double (&kk)[E], int (&ii)[E]) {
-  const int lane = threadIdx.x & (kWave - 1);
-  if constexpr (STRIDE >= kWave) {
-    constexpr int R = STRIDE / kWave;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if ((e & R) == 0) {
-        const int f = e | R;
-        const bool asc = ((e * kWave) & SIZE) == 0;
-        const bool sw = asc ? nn_less(kk[f], ii[f], kk[e], ii[e]) : nn_less(kk[e], ii[e], kk[f], ii[f]);
-        const double ka = kk[e];
-        const int ia = ii[e];
-        kk[e] = sw ? kk[f] : kk[e];
-        ii[e] = sw ? ii[f] : ii[e];
-        kk[f] = sw ? ka : kk[f];
-        ii[f] = sw ? ia : ii[f];
-      }
-    }
-  } else {
-    const bool lo = (lane & STRIDE) == 0;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const double k2 = xchg<STRIDE>(kk[e]);
-      const int i2 = xchg<STRIDE>(ii[e]);
-      const bool asc = ((e * kWave + lane) & SIZE) == 0;
-      const bool eq = k2 == kk[e];
-      const bool pless = (k2 < kk[e]) | (eq & (i2 < ii[e]));
-      const bool pmore = (k2 > kk[e]) | (eq & (i2 > ii[e]));
-      const bool take = (asc == lo) ? pless : pmore;
-      kk[e] = take ? k2 : kk[e];
-      ii[e] = take ? i2 : ii[e];
-    }
-  }
-  if constexpr (STRIDE > 1) knn_sort_stage<E, SIZE, STRIDE / 2>(kk, ii);
-}
-
-template <int E, int SIZE>
-__device__ __forceinline__ void knn_sort(double (&kk)[E], int (&ii)[E]) {
-  knn_sort_stage<E, SIZE, SIZE / 2>(kk, ii);
-  if constexpr (SIZE < E * kWave) knn_sort<E, SIZE * 2>(kk, ii);
-}
-
 template <int DM, int LCAP>
 __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
     const KNNDesc* __restrict__ descs) {
@@ -841,7 +940,7 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
       kk[e] = key[e * kWave + lane];
       ii[e] = idx[e * kWave + lane];
     }
-    knn_sort<E, 2>(kk, ii);
+    reg_sort<MinFirst, E>(kk, ii);
     wave_sync();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -971,13 +1070,14 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   double* s2 = ctx->arena.get_t<double>("mm.s2", tot);
   double* dist = ctx->arena.get_t<double>("mm.dist", tot);
   int* acc = ctx->arena.get_t<int>("mm.acc", (size_t)nd * kAccCap);
+  double* accx = ctx->arena.get_t<double>("mm.accx", (size_t)nd * kAccCap * kMMD);
   MMState* st = ctx->arena.get_t<MMState>("mm.st", nd);
   double* ck = ctx->arena.get_t<double>("mm.ck", (size_t)nslice_tot * (kSliceTop + 1));
   int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTop + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
-                     s2, dist, acc, st, dorder);
+                     dXT, s2, dist, acc, accx, st, dorder);
   SVH_HIP(hipGetLastError());
   if (maxn <= 1) return;
   SVH_REQUIRE(dmax <= kMMD, "maximin: d above 12 not supported");
@@ -1009,24 +1109,24 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
       const dim3 ba(kSlice * kApplyParts);
       if (dm == 4)
         hipLaunchKernelGGL(mm_apply_kernel<4>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, st, ck, ci);
+                           acc, accx, st, ck, ci);
       else if (dm == 8)
         hipLaunchKernelGGL(mm_apply_kernel<8>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, st, ck, ci);
+                           acc, accx, st, ck, ci);
       else
         hipLaunchKernelGGL(mm_apply_kernel<12>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, st, ck, ci);
+                           acc, accx, st, ck, ci);
       ctx->timers.end(ctx->stream);
       ctx->timers.begin(ctx->stream, "maximin_select");
       if (dm == 4)
         hipLaunchKernelGGL(mm_select_kernel<4>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
+                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
       else if (dm == 8)
         hipLaunchKernelGGL(mm_select_kernel<8>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
+                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
       else
         hipLaunchKernelGGL(mm_select_kernel<12>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, st, dorder, sortcap, mprof);
+                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
       ctx->timers.end(ctx->stream);
     }
     SVH_HIP(hipGetLastError());
