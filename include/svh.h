@@ -171,6 +171,21 @@ int svh_batch_setup(svh_ctx* ctx, int32_t ndata, const int32_t* d, const int32_t
                     const int32_t* fixed_nug, const double* log_nug, double lambda_max);
 int svh_batch_update(svh_ctx* ctx, int32_t i, const double* X, const double* y, const double* Z,
                      const int64_t* nbr, const int32_t* kcnt);
+
+/* The unordered estimation data of every dataset of the batch (Xe, ye, Ze in
+ * the layout of svh_batch_setup's X, y, Z), kept on the device for
+ * svh_batch_rebuild.  Call after svh_batch_setup. */
+int svh_batch_data(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze);
+
+/* Rebuild of `count` datasets of the batch (indices which[k]) on the device:
+ * the data of svh_batch_data scaled by lam[k * 12 + l] (d positive ranges),
+ * maximin order from first[k], ordered m[k] nearest neighbours, and the
+ * ordered rows, neighbour lists and counts written where the fitting kernels
+ * read them -- svh_batch_update without the host round trip.  Replaces the
+ * rebuild of gp.py:182-191 for each dataset (d <= 10, m[k] <= the batch's
+ * mmax). */
+int svh_batch_rebuild(svh_ctx* ctx, int32_t count, const int32_t* which, const double* lam,
+                      const int64_t* first, const int32_t* m);
 int svh_batch_fit_start(svh_ctx* ctx, const double* theta0, int32_t max_iter, double tol,
                         double var_penalty, int32_t use_graph);
 int svh_batch_fit_step(svh_ctx* ctx, int32_t nrep, int32_t* status);

@@ -73,6 +73,8 @@ def _load():
                                   _i64p, _i32p, _i32p, _i32p, ctypes.c_int32, _dp, _i32p, _dp,
                                   ctypes.c_double]
     L.svh_batch_update.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _i64p, _i32p]
+    L.svh_batch_data.argtypes = [_vp, _dp, _dp, _dp]
+    L.svh_batch_rebuild.argtypes = [_vp, ctypes.c_int32, _i32p, _dp, _i64p, _i32p]
     L.svh_batch_fit_start.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int32]
     L.svh_batch_fit_step.argtypes = [_vp, ctypes.c_int32, _i32p]
@@ -98,7 +100,8 @@ def _load():
                  "svh_mlp_forward", "svh_mlp_loss_grad", "svh_mlp_load_data",
                  "svh_mlp_train_epoch", "svh_simulate_rff"):
         getattr(L, name).restype = ctypes.c_int
-    for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_fit_start", "svh_batch_fit_step",
+    for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_data",
+                 "svh_batch_rebuild", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
                  "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
                  "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
@@ -115,7 +118,7 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_batch_eval", "svh_ordering_batch", "svh_mlp_init", "svh_mlp_num_params",
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
-            "svh_autotune_read", "svh_vecchia_rebuild")
+            "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild")
 
 
 def device_count() -> int:
@@ -452,6 +455,26 @@ class BatchHandle:
         c._check(self._L.svh_batch_update(c._h, int(i), _ptr(_finite(X, "X")),
                                           _ptr(_finite(y, "y")), _ptr(Z), _ptr(nbr, _i64p),
                                           _ptr(kcnt, _i32p)))
+
+    def data(self, Xe, ye, Ze):
+        """Unordered estimation data of every dataset (setup layout), kept on
+        the device for `rebuild`."""
+        self._data = (_finite(Xe, "Xe"), _finite(ye, "ye"),
+                      _f64(Ze) if Ze is not None and Ze.size else np.zeros(1))
+        c = self.ctx
+        c._check(self._L.svh_batch_data(c._h, _ptr(self._data[0]), _ptr(self._data[1]),
+                                        _ptr(self._data[2])))
+
+    def rebuild(self, which, lams, firsts, ms):
+        """Device rebuild of datasets `which` at ranges lams[k] (svh_batch_rebuild)."""
+        which = _i32(which)
+        k = len(which)
+        lam = np.ones((k, 12))
+        for r, la in enumerate(lams):
+            lam[r, :len(la)] = la
+        c = self.ctx
+        c._check(self._L.svh_batch_rebuild(c._h, k, _ptr(which, _i32p), _ptr(lam),
+                                           _ptr(_i64(firsts), _i64p), _ptr(_i32(ms), _i32p)))
 
     def fit_start(self, theta0, max_iter, tol, var_penalty, use_graph=True):
         th = _f64(theta0)

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from .gp import ScaledVecchiaGP
-from .likelihood import _chunk_ids
+from .likelihood import _chunk_ids, _chunk_ids_rows
 from .ordering import _nn_groups
 
 __all__ = ["fit_batch", "BatchedFit"]
@@ -83,26 +83,30 @@ class BatchedFit:
                         th[-1] = math.log(models[i].nugget)
                     preps[i]["theta"] = models[i]._clip_theta(th, preps[i]["d"])
         mmax = max(p["m"] for p in preps)
-        structs = self._structures(ctx, models, preps, [p["theta"] for p in preps], mmax)
-        t_build = time.perf_counter() - t0
-        # concatenate
+        # the batch is set up with the unordered data and empty neighbour
+        # lists; every structure is then built on the device
         d = np.array([p["d"] for p in preps], np.int32)
-        q = np.array([s["Zo"].shape[1] for s in structs], np.int32)
+        q = np.array([p["Ze"].shape[1] for p in preps], np.int32)
         nu = np.array([_NU_CODE[g.nu] for g in models], np.int32)
-        n = np.array([len(s["yo"]) for s in structs], np.int64)
-        X = np.concatenate([s["Xo"].ravel() for s in structs])
-        y = np.concatenate([s["yo"] for s in structs])
-        Z = np.concatenate([s["Zo"].ravel() for s in structs]) if q.sum() else np.zeros(0)
-        nbr = np.concatenate([s["nbr"] for s in structs])
-        kcnt = np.concatenate([s["kcnt"] for s in structs])
-        chunk = np.concatenate([s["chunk"] for s in structs])
-        nchunk = np.array([s["nchunk"] for s in structs], np.int32)
+        n = np.array([len(p["ye"]) for p in preps], np.int64)
+        X = np.concatenate([np.ascontiguousarray(p["Xe"]).ravel() for p in preps])
+        y = np.concatenate([p["ye"] for p in preps])
+        Z = (np.concatenate([np.ascontiguousarray(p["Ze"]).ravel() for p in preps]) if q.sum()
+             else np.zeros(0))
+        nbr = np.full((int(n.sum()), mmax), -1, np.int64)
+        kcnt = np.zeros(int(n.sum()), np.int32)
+        chunks = [_chunk_ids_rows(len(p["ye"]), p["m"], p["d"] + 2) for p in preps]
+        chunk = np.concatenate([c for c, _ in chunks])
+        nchunk = np.array([nc for _, nc in chunks], np.int32)
         lvt = np.array([p["lvt"] for p in preps])
         fixed = np.array([g.nugget is not None for g in models], np.int32)
         lnug = np.array([math.log(g.nugget) if g.nugget is not None else 0.0 for g in models])
         bh = _lib.BatchHandle(ctx)
         bh.setup(d, q, nu, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt, fixed, lnug,
                  models[0].lambda_max)
+        bh.data(X, y, Z)
+        self._rebuild(bh, preps, list(range(nd)), [p["theta"] for p in preps])
+        t_build = time.perf_counter() - t0
         theta0 = np.zeros((nd, _lib.BP))
         for i, p in enumerate(preps):
             theta0[i, :p["d"] + 2] = p["theta"]
@@ -125,11 +129,7 @@ class BatchedFit:
                     preps[i]["theta"] = preps[i]["theta_default"].copy()
                     theta0[i, :preps[i]["d"] + 2] = preps[i]["theta"]
                     use_ws[i] = False
-                new = self._structures(ctx, [models[i] for i in back], [preps[i] for i in back],
-                                       [preps[i]["theta"] for i in back], mmax)
-                for i, st_i in zip(back, new):
-                    structs[i] = st_i
-                    bh.update(i, st_i["Xo"], st_i["yo"], st_i["Zo"], st_i["nbr"], st_i["kcnt"])
+                self._rebuild(bh, preps, back, [preps[i]["theta"] for i in back])
         last = [np.exp(p["theta"][1:1 + p["d"]]).copy() for p in preps]
         bh.fit_start(theta0, models[0].max_iter, models[0].tol, models[0].var_penalty,
                      self.use_graph)
@@ -156,11 +156,8 @@ class BatchedFit:
                     redo.append(i)
                     last[i] = r.copy()
                 if redo:
-                    new = self._structures(ctx, [models[i] for i in redo], [preps[i] for i in redo],
-                                           [th[i, :preps[i]["d"] + 2] for i in redo], mmax)
-                    for i, s in zip(redo, new):
-                        structs[i] = s
-                        bh.update(i, s["Xo"], s["yo"], s["Zo"], s["nbr"], s["kcnt"])
+                    self._rebuild(bh, preps, redo, [th[i, :preps[i]["d"] + 2] for i in redo])
+                    for i in redo:
                         bh.set_reorder(i, True)
                         rebuilds[i] += 1
                 t_reorder += time.perf_counter() - tr
@@ -195,6 +192,21 @@ class BatchedFit:
                           build_s=t_build, loop_s=t_loop, reorder_s=t_reorder,
                           evals=int(nev.sum()), rebuilds=int(sum(rebuilds)))
         return models
+
+    @staticmethod
+    def _rebuild(bh, preps, which, thetas):
+        """Device rebuild of datasets `which` at `thetas` (maximin order and
+        ordered neighbours of the scaled inputs, gp.py:182-191), the first
+        point chosen on the host as ordering.py:72 does."""
+        lams, firsts, ms = [], [], []
+        for i, th in zip(which, thetas):
+            p = preps[i]
+            lam = np.exp(np.asarray(th, float)[1:1 + p["d"]])
+            Xs = p["Xe"] / lam
+            firsts.append(int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1))))
+            lams.append(lam)
+            ms.append(p["m"])
+        bh.rebuild(which, lams, firsts, ms)
 
     @staticmethod
     def _structures(ctx, models, preps, thetas, mmax):

@@ -311,6 +311,23 @@ int svh_batch_update(svh_ctx* ctx, int32_t i, const double* X, const double* y, 
   });
 }
 
+int svh_batch_data(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(Xe && ye, "batch_data: null pointer");
+    SVH_REQUIRE(batch_layout(ctx, 0).nz == 0 || Ze, "batch_data: Ze is NULL with q > 0");
+    batch_data_device(ctx, Xe, ye, Ze);
+  });
+}
+
+int svh_batch_rebuild(svh_ctx* ctx, int32_t count, const int32_t* which, const double* lam,
+                      const int64_t* first, const int32_t* m) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(count >= 0 && (count == 0 || (which && lam && first && m)),
+                "batch_rebuild: null pointer");
+    batch_rebuild_device(ctx, count, which, lam, first, m);
+  });
+}
+
 int svh_batch_fit_start(svh_ctx* ctx, const double* theta0, int32_t max_iter, double tol,
                         double var_penalty, int32_t use_graph) {
   return guarded(ctx, [&] {

@@ -1807,14 +1807,6 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
 // k = min(i, m) are consecutive rows, so blocks run in row order); the jitter
 // chunk of block i is i below m (one-row groups) and m + (i - m) / step in the
 // k = m group (steps for the gradient and value passes).
-__global__ void gather_soa(const double* __restrict__ XT, const int64_t* __restrict__ order,
-                           int64_t n, int d, double* __restrict__ XTo) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n * d) return;
-  const int64_t l = t / n, i = t - l * n;
-  XTo[t] = XT[l * n + order[i]];
-}
-
 __global__ void rebuild_rows(const double* __restrict__ Xe, const double* __restrict__ ye,
                              const double* __restrict__ Ze, const int64_t* __restrict__ order,
                              const int64_t* __restrict__ nn, int64_t n, int d, int q, int m,
@@ -1871,10 +1863,7 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   // X / lambda in SoA (the same correctly rounded quotients as the host's)
   launch_to_soa(ctx, dXe, n, d, dlam, dXT);
   launch_maximin(ctx, dXT, n, d, first, dord);
-  const int64_t tot = n * d;
-  hipLaunchKernelGGL(gather_soa, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dXT, dord,
-                     n, d, dXTo);
-  SVH_HIP(hipGetLastError());
+  launch_gather_soa(ctx, dXT, dord, n, d, dXTo);
   launch_ordered_knn(ctx, dXTo, n, d, m, 0, dnn);
   const int P = d + 2, K = m + 1, mcol = m > 0 ? m : 1;
   const int64_t stepg = std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K * P));
@@ -2166,6 +2155,7 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
                         const int32_t* chunk, const int32_t* nchunk, int mmax, const double* lvt,
                         const int32_t* fixed_nug, const double* log_nug, double lambda_max) {
   batch_release(ctx);
+  ctx->batch_data = false;
   BatchState& B = batch_of(ctx);
   SVH_REQUIRE(ndata >= 1, "batch: no datasets");
   SVH_REQUIRE(mmax >= 1 && mmax + 1 <= 32, "batch: conditioning sets must have at most 31 points");
@@ -2299,6 +2289,26 @@ void batch_update_device(svh_ctx* ctx, int i, const double* X, const double* y, 
                          hipMemcpyHostToDevice, s));
   SVH_HIP(hipMemcpyAsync(dk + D.blk0, hk.data(), hk.size() * sizeof(int), hipMemcpyHostToDevice, s));
   SVH_HIP(hipStreamSynchronize(s));
+}
+
+// Offsets of dataset i in the batch buffers b.* (for device-side rebuilds).
+BatchLayout batch_layout(svh_ctx* ctx, int i) {
+  BatchState& B = batch_of(ctx);
+  SVH_REQUIRE(B.ready && i >= 0 && i < B.ndata, "batch: bad dataset");
+  const DSDesc& D = B.desc[i];
+  BatchLayout L;
+  L.xoff = D.xoff;
+  L.yoff = D.yoff;
+  L.zoff = D.zoff;
+  L.blk0 = D.blk0;
+  L.n = D.nblk;
+  L.d = D.d;
+  L.q = D.q;
+  L.mmax = B.mmax;
+  L.nx = B.nx;
+  L.ny = B.ny;
+  L.nz = B.nz;
+  return L;
 }
 
 // Enqueue one evaluation of every evaluating dataset plus the state update.

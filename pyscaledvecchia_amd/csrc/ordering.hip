@@ -11,6 +11,7 @@
 #include "svh_internal.h"
 
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -1052,6 +1053,23 @@ void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const doubl
   SVH_HIP(hipGetLastError());
 }
 
+__global__ void gather_soa_kernel(const double* __restrict__ XT, const int64_t* __restrict__ order,
+                                  int64_t n, int d, double* __restrict__ XTo) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * d) return;
+  const int64_t l = t / n, i = t - l * n;
+  XTo[t] = XT[l * n + order[i]];
+}
+
+void launch_gather_soa(svh_ctx* ctx, const double* dXT, const int64_t* dorder, int64_t n, int d,
+                       double* dXTo) {
+  const int64_t tot = n * d;
+  if (tot == 0) return;
+  hipLaunchKernelGGL(gather_soa_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     ctx->stream, dXT, dorder, n, d, dXTo);
+  SVH_HIP(hipGetLastError());
+}
+
 // Maximin orders of several problems at once: rounds of (apply, select)
 // launches over all unfinished problems until every order is complete.
 void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMDesc>& hd,
@@ -1243,6 +1261,115 @@ void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t
   }
   launch_knn_descs(ctx, hk);
   SVH_HIP(hipMemcpyAsync(nn, dnn, (size_t)so * (m + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  ctx->timers.collect();
+}
+
+// ------------------------------------------------------------------ batched rebuilds
+// The unordered estimation data of every batched dataset stay on the device
+// (b.Xe, b.ye, b.Ze, in the layout of b.X, b.y, b.Z).  A rebuild of selected
+// datasets scales them by their ranges, orders them (one batched maximin),
+// finds the ordered neighbours (one batched kNN) and writes the ordered rows,
+// neighbour lists and counts into the batch buffers the fitting kernels read.
+
+__global__ void batch_rows_kernel(const double* __restrict__ Xe, const double* __restrict__ ye,
+                                  const double* __restrict__ Ze, const int64_t* __restrict__ order,
+                                  const int64_t* __restrict__ nn, int64_t n, int d, int q, int m,
+                                  int mmax, double* __restrict__ X, double* __restrict__ y,
+                                  double* __restrict__ Z, int* __restrict__ nbr,
+                                  int* __restrict__ kcnt) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t src = order[r];
+  for (int l = 0; l < d; ++l) X[r * d + l] = Xe[src * d + l];
+  y[r] = ye[src];
+  for (int c = 0; c < q; ++c) Z[r * q + c] = Ze[src * q + c];
+  const int k = (int)(r < m ? r : m);
+  kcnt[r] = k;
+  for (int t = 0; t < mmax; ++t) nbr[r * mmax + t] = t < k ? (int)nn[r * (mmax + 1) + 1 + t] : -1;
+}
+
+void batch_data_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze) {
+  const BatchLayout L = batch_layout(ctx, 0);
+  hipStream_t s = ctx->stream;
+  double* dX = ctx->arena.get_t<double>("b.Xe", L.nx);
+  double* dy = ctx->arena.get_t<double>("b.ye", L.ny);
+  double* dZ = ctx->arena.get_t<double>("b.Ze", L.nz > 0 ? L.nz : 1);
+  SVH_HIP(hipMemcpyAsync(dX, Xe, (size_t)L.nx * sizeof(double), hipMemcpyHostToDevice, s));
+  SVH_HIP(hipMemcpyAsync(dy, ye, (size_t)L.ny * sizeof(double), hipMemcpyHostToDevice, s));
+  if (L.nz > 0)
+    SVH_HIP(hipMemcpyAsync(dZ, Ze, (size_t)L.nz * sizeof(double), hipMemcpyHostToDevice, s));
+  SVH_HIP(hipStreamSynchronize(s));
+  ctx->batch_data = true;
+}
+
+void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const double* lam,
+                          const int64_t* first, const int32_t* m) {
+  if (count <= 0) return;
+  SVH_REQUIRE(ctx->batch_data, "batch rebuild: call svh_batch_data after svh_batch_setup");
+  std::vector<BatchLayout> lay(count);
+  std::vector<MMDesc> hm(count);
+  int64_t xo = 0, so = 0;
+  int sl = 0, mmax = 0;
+  for (int k = 0; k < count; ++k) {
+    const BatchLayout L = batch_layout(ctx, which[k]);
+    SVH_REQUIRE(L.d <= kMMD, "batch rebuild: d above 12");
+    SVH_REQUIRE(m[k] >= 0 && m[k] <= L.mmax && m[k] < L.n, "batch rebuild: bad m");
+    SVH_REQUIRE(first[k] >= 0 && first[k] < L.n, "batch rebuild: first out of range");
+    for (int l = 0; l < L.d; ++l)
+      SVH_REQUIRE(std::isfinite(lam[k * kMMD + l]) && lam[k * kMMD + l] > 0.0,
+                  "batch rebuild: ranges must be positive");
+    lay[k] = L;
+    MMDesc& D = hm[k];
+    D.xoff = xo;
+    D.soff = so;
+    D.n = (int)L.n;
+    D.d = L.d;
+    D.first = (int)first[k];
+    D.slice0 = sl;
+    D.nslice = (D.n + kSlice - 1) / kSlice;
+    D.pad = 0;
+    xo += L.n * L.d;
+    so += L.n;
+    sl += D.nslice;
+    mmax = L.mmax;
+  }
+  hipStream_t s = ctx->stream;
+  const double* dXe = (const double*)ctx->arena.get("b.Xe", 0);
+  const double* dye = (const double*)ctx->arena.get("b.ye", 0);
+  const double* dZe = (const double*)ctx->arena.get("b.Ze", 0);
+  SVH_REQUIRE(ctx->batch_data, "batch rebuild: call svh_batch_data after svh_batch_setup");
+  double* dlam = ctx->arena.get_t<double>("br.lam", (size_t)count * kMMD);
+  double* dXT = ctx->arena.get_t<double>("br.XT", xo);
+  double* dXTo = ctx->arena.get_t<double>("br.XTo", xo);
+  int64_t* dord = ctx->arena.get_t<int64_t>("br.order", so);
+  int64_t* dnn = ctx->arena.get_t<int64_t>("br.nn", (size_t)so * (mmax + 1));
+  SVH_HIP(hipMemcpyAsync(dlam, lam, (size_t)count * kMMD * sizeof(double), hipMemcpyHostToDevice,
+                         s));
+  for (int k = 0; k < count; ++k)
+    launch_to_soa(ctx, dXe + lay[k].xoff, lay[k].n, lay[k].d, dlam + k * kMMD, dXT + hm[k].xoff);
+  launch_maximin_batch(ctx, dXT, hm, dord);
+  std::vector<KNNDesc> hk(count);
+  for (int k = 0; k < count; ++k) {
+    launch_gather_soa(ctx, dXT + hm[k].xoff, dord + hm[k].soff, lay[k].n, lay[k].d,
+                      dXTo + hm[k].xoff);
+    hk[k] = KNNDesc{dXTo + hm[k].xoff, nullptr, dnn + hm[k].soff * (mmax + 1), lay[k].n, 0, 1,
+                    lay[k].d, m[k], 1, mmax + 1};
+  }
+  launch_knn_descs(ctx, hk);
+  double* bX = (double*)ctx->arena.get("b.X", 0);
+  double* by = (double*)ctx->arena.get("b.y", 0);
+  double* bZ = (double*)ctx->arena.get("b.Z", 0);
+  int* bnbr = (int*)ctx->arena.get("b.nbr", 0);
+  int* bk = (int*)ctx->arena.get("b.kcnt", 0);
+  for (int k = 0; k < count; ++k) {
+    const BatchLayout& L = lay[k];
+    hipLaunchKernelGGL(batch_rows_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s,
+                       dXe + L.xoff, dye + L.yoff, dZe + L.zoff, dord + hm[k].soff,
+                       dnn + hm[k].soff * (mmax + 1), L.n, L.d, L.q, m[k], mmax, bX + L.xoff,
+                       by + L.yoff, bZ + L.zoff, bnbr + L.blk0 * mmax, bk + L.blk0);
+    SVH_HIP(hipGetLastError());
+  }
   SVH_HIP(hipStreamSynchronize(s));
   ctx->timers.collect();
 }

@@ -175,6 +175,7 @@ struct svh_ctx {
   int mm_rounds = 0;  // rounds of the last maximin call
   int64_t rb_n = 0;   // sizes of the estimation data resident for rebuilds
   int rb_d = 0, rb_q = 0;
+  bool batch_data = false;   // unordered batch data uploaded for the current batch
   MlpState* mlp = nullptr;
   Autotune tune;
   std::string last_error;
@@ -190,6 +191,20 @@ void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const do
                       int64_t nq, int d, int k, int64_t* dnb, int nb_stride);
 void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const double* dlam,
                    double* dXT);
+// XTo[l][i] = XT[l][order[i]] (SoA, n x d)
+void launch_gather_soa(svh_ctx* ctx, const double* dXT, const int64_t* dorder, int64_t n, int d,
+                       double* dXTo);
+// rebuilds of selected datasets of the batch on the device
+void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const double* lam,
+                          const int64_t* first, const int32_t* m);
+void batch_data_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze);
+// offsets of one dataset in the batch buffers
+struct BatchLayout {
+  int64_t xoff, yoff, zoff, blk0, n;
+  int d, q, mmax;
+  int64_t nx, ny, nz;
+};
+BatchLayout batch_layout(svh_ctx* ctx, int i);
 // loglik.hip
 void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
                             int64_t n, int d, int q, const double* lam, int64_t first, int m,

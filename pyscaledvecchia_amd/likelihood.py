@@ -31,6 +31,18 @@ def _chunk_ids(groups, P):
     return (np.concatenate(ids) if ids else np.zeros(0, np.int64)), base
 
 
+def _chunk_ids_rows(n, m, P):
+    """_chunk_ids of the groups _nn_groups forms for the ordered neighbour
+    lists of n rows with m neighbours (ordering.py:119-127): the groups of
+    k = min(i, m) are rows 0..m-1 one by one, then rows m..n-1 together, so
+    the chunk of row i is i below m and m + (i - m) // step above."""
+    K = m + 1
+    step = max(1, _CHUNK_BUDGET // max(1, K * K * max(P, 1)))
+    i = np.arange(n)
+    ids = np.where(i < m, i, m + (i - m) // step)
+    return ids, (m + (n - m + step - 1) // step) if n > m else n
+
+
 class VecchiaStructure:
     """Device-resident Vecchia structure: ordered inputs, responses, design
     and neighbour groups (the state gp.py:182-191 rebuilds)."""

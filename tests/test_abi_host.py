@@ -69,6 +69,30 @@ def test_chunk_ids_follow_reference_slicing(P):
     assert nchunk == c
 
 
+@pytest.mark.parametrize("n,m,P", [(1, 0, 3), (2, 1, 3), (10, 3, 5), (3000, 20, 7),
+                                   (5000, 30, 1), (50000, 30, 10), (100, 99, 12),
+                                   (200000, 140, 12)])
+def test_row_chunk_ids_match_group_chunk_ids(n, m, P):
+    """The closed form used by the device rebuilds (chunk i below m, then
+    m + (i - m) // step) equals the reference slicing of the k = min(i, m)
+    groups (likelihood.py:111-114, _covariance.py:255-259)."""
+    from pyscaledvecchia_amd.likelihood import _chunk_ids_rows
+    rows = np.arange(n)
+    ks = np.minimum(rows, m)
+    groups = [(rows[ks == k], np.zeros((int((ks == k).sum()), k), np.int64))
+              for k in np.unique(ks)]
+    ref = []
+    c = 0
+    for r, nb in groups:
+        K = nb.shape[1] + 1
+        for a0, a1 in O.chunks(len(r), K, P):
+            ref += [c] * (a1 - a0)
+            c += 1
+    ids, nchunk = _chunk_ids_rows(n, m, P)
+    np.testing.assert_array_equal(ids, ref)
+    assert nchunk == c
+
+
 def test_nn_groups_matches_oracle():
     from pyscaledvecchia_amd.ordering import _nn_groups
     nn = O.find_ordered_nn(np.random.default_rng(1).random((80, 2)), 6)
