@@ -159,6 +159,70 @@ def pred_block_flops(K, d):
     return T * (3 * d + 6) + K ** 3 / 3 + K ** 2 + 2 * K
 
 
+def run_c5j(args):
+    """Joint prediction and sampling on a path: sample_joint with n_sim = 100
+    on n_test points after n_train observations (d = 10, m_pred = 140),
+    predicted from a fixed fitted state; the solves run level by level on the
+    device.  The CPU baseline times the reference's host solves
+    (scipy spsolve_triangular) on the same factor."""
+    from scipy import sparse
+    from scipy.sparse.linalg import spsolve_triangular
+    from pyscaledvecchia_amd import ScaledVecchiaGP, _lib
+    ctx = _lib.context()
+    d, m, nsim = 10, 140, 100
+    X, y, Xs = c5_inputs(args.n_train, args.n_test, d)
+    gp = ScaledVecchiaGP(m_est=30, m_pred=m, nu=2.5, trend="constant")
+    # fitted state: standardised inputs and responses, fixed parameters
+    gp._lo, gp._span = X.min(0), X.max(0) - X.min(0)
+    gp._ymu, gp._ysd = y.mean(), y.std()
+    gp.X_, gp.y_ = (X - gp._lo) / gp._span, (y - gp._ymu) / gp._ysd
+    gp.d_ = d
+    gp.theta_ = np.concatenate([[0.0], np.full(d, math.log(0.5)), [math.log(1e-4)]])
+    gp.beta_ = np.array([0.0])
+    gp.b_ = 1.0
+    gp._estimate_nu = False
+    gp._nu_fit = 2.5
+    gp.predict_joint(Xs[:500], n_sim=4, random_state=0)        # warm-up
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    out = gp.predict_joint(Xs, n_sim=nsim, random_state=1)
+    t_total = time.perf_counter() - t0
+    ctx.timing(False)
+    Xsn = (Xs - gp._lo) / gp._span
+    t0 = time.perf_counter()
+    block, av, perm, order_obs = gp._joint_factor(Xsn, m)
+    t_factor = time.perf_counter() - t0
+    ns, K, nobs = block.shape[0], block.shape[1], gp.X_.shape[0]
+    resid = gp.y_ - gp.beta_[0]
+    eps = np.random.default_rng(2).standard_normal((ns, nsim))
+    t0 = time.perf_counter()
+    mu, x, _ = ctx.joint_solve(nobs, block, av, resid=resid[order_obs], rhs=eps)
+    t_solve = time.perf_counter() - t0
+    # reference host solves on the same factor (CPU baseline, one core)
+    U = sparse.coo_matrix((av.ravel(), (block.ravel(), np.repeat(np.arange(ns), K))),
+                          shape=(nobs + ns, ns)).tocsc()
+    Lt = U[nobs:, :].T.tocsr()
+    t0 = time.perf_counter()
+    ref_mu = spsolve_triangular(Lt, -(U[:nobs, :].T @ resid[order_obs]), lower=True)
+    ref_x = spsolve_triangular(Lt, eps, lower=True)
+    t_cpu = time.perf_counter() - t0
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))   # noqa: E731
+    print(json.dumps({
+        "metric": "sample_joint seconds, n_sim=100 on a path, m_pred=140, d=10",
+        "value": t_total, "unit": "s", "n_gpus": 1, "steps": 1, "higher_is_better": False,
+        "dtype": "f64", "vs_baseline": None, "data": "synthetic (GMM design, sin-sum function)",
+        "config": {"workload": f"C5-shaped joint sampling n_train={args.n_train} "
+                               f"path={args.n_test} n_sim={nsim} m_pred={m} d={d}"},
+        "breakdown_s": {"factor (orderings, kNN, blocks)": t_factor,
+                        "device solves (mean + 100 paths)": t_solve},
+        "solve_vs_scipy_rel_err": {"mean": rel(mu, ref_mu), "paths": rel(x, ref_x)},
+        "cpu_baseline": {"value": t_cpu, "unit": "s", "cores": 1, "kind": "reference",
+                         "sample": "scipy spsolve_triangular (the reference's host solves, "
+                                   "gp.py:380-401) on the same factor: mean + 100 paths"},
+        "samples_shape": list(out["samples"].shape)}))
+
+
 def run_c5p(args):
     """Marginal prediction at m_pred = 140 (K = 141 blocks): kNN query plus the
     matrix-core block kernel; inputs resident before the timed region."""
@@ -249,7 +313,7 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5p"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5p", "c5j"])
     ap.add_argument("--n-train", type=int, default=50_000)
     ap.add_argument("--n-test", type=int, default=20_000)
     ap.add_argument("--n-c4", type=int, default=50_000)
@@ -266,6 +330,11 @@ def main():
         return
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "c5j":
+        if args.n_test == 20_000:
+            args.n_test = 10_000
+        run_c5j(args)
         return
     if args.workload == "c5p":
         if args.steps == 50:

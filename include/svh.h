@@ -142,6 +142,21 @@ int svh_block_columns(svh_ctx* ctx, const double* Xall, int64_t nall, int32_t d,
                       const int64_t* blocks, const double* mask, int64_t nblk, int32_t K,
                       const double* theta, double nu, int32_t precision, double* a);
 
+/* Joint prediction solves with the factor of svh_block_columns: `blocks`
+ * (ns x K, indices into the observed-first ordering, -1 for none; the last
+ * entry of block i is nobs + i) and its values a (ns x K).  Lt = U_pp^T has
+ * row i = block i's prediction-side entries, the diagonal last.
+ *   mean (ns, may be NULL): Lt^-1 b with b_i = -sum over the observed
+ *       entries of a_ik resid[block[i,k]] (resid: nobs, in observed order);
+ *   x (ns x nrhs, row-major): Lt^-1 rhs for nrhs right-hand sides;
+ *   var (ns, when exact_var): row sums of squares of Lt^-1.
+ * Solved level by level on the device (a row's level = 1 + the deepest level
+ * among its prediction neighbours).  Replaces the spsolve_triangular calls
+ * of gp.py:380-401 (predict_joint / sample_joint / exact_var). */
+int svh_joint_solve(svh_ctx* ctx, int64_t nobs, int64_t ns, int32_t K, const int64_t* blocks,
+                    const double* a, const double* resid, int32_t nrhs, const double* rhs,
+                    double* mean, double* x, int32_t exact_var, double* var);
+
 /* ------------------------------------------------------------ batched --- */
 /* Many independent fits at once (no counterpart in the reference, which fits
  * one dataset per ScaledVecchiaGP.fit, gp.py:138-239).  All blocks of all

@@ -74,6 +74,8 @@ def _load():
                                   ctypes.c_double]
     L.svh_batch_update.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _dp, _i64p, _i32p]
     L.svh_batch_data.argtypes = [_vp, _dp, _dp, _dp]
+    L.svh_joint_solve.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _i64p, _dp,
+                                  _dp, ctypes.c_int32, _dp, _dp, _dp, ctypes.c_int32, _dp]
     L.svh_batch_rebuild.argtypes = [_vp, ctypes.c_int32, _i32p, _dp, _i64p, _i32p]
     L.svh_batch_fit_start.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int32]
@@ -101,7 +103,7 @@ def _load():
                  "svh_mlp_train_epoch", "svh_simulate_rff"):
         getattr(L, name).restype = ctypes.c_int
     for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_data",
-                 "svh_batch_rebuild", "svh_batch_fit_start", "svh_batch_fit_step",
+                 "svh_batch_rebuild", "svh_joint_solve", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
                  "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
                  "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
@@ -118,7 +120,8 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_batch_eval", "svh_ordering_batch", "svh_mlp_init", "svh_mlp_num_params",
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
-            "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild")
+            "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild",
+            "svh_joint_solve")
 
 
 def device_count() -> int:
@@ -328,6 +331,26 @@ class Context:
             self._h, _ptr(Xall), Xall.shape[0], Xall.shape[1], _ptr(blocks, _i64p), _ptr(mask),
             nblk, K, _ptr(theta), float(nu), PRECISION[precision], _ptr(a)))
         return a
+
+    def joint_solve(self, nobs, blocks, a, resid=None, rhs=None, exact_var=False):
+        """Solves with Lt = U_pp^T of the joint factor (svh_joint_solve).
+        Returns (mean or None, Lt^-1 rhs or None, exact variances or None)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.int64)
+        a = _f64(a)
+        ns, K = blocks.shape
+        mean = np.empty(ns) if resid is not None else None
+        resid = None if resid is None else _f64(resid)
+        nrhs = 0
+        x = None
+        if rhs is not None:
+            rhs = _f64(rhs).reshape(ns, -1)
+            nrhs = rhs.shape[1]
+            x = np.empty((ns, nrhs))
+        var = np.empty(ns) if exact_var else None
+        self._check(self._L.svh_joint_solve(
+            self._h, int(nobs), ns, K, _ptr(blocks, _i64p), _ptr(a), _ptr(resid), nrhs,
+            _ptr(rhs), _ptr(mean), _ptr(x), 1 if exact_var else 0, _ptr(var)))
+        return mean, x, var
 
 
 BP, BQ = 12, 11   # parameter / trend strides of the batched entry points

@@ -311,6 +311,16 @@ int svh_batch_update(svh_ctx* ctx, int32_t i, const double* X, const double* y, 
   });
 }
 
+int svh_joint_solve(svh_ctx* ctx, int64_t nobs, int64_t ns, int32_t K, const int64_t* blocks,
+                    const double* a, const double* resid, int32_t nrhs, const double* rhs,
+                    double* mean, double* x, int32_t exact_var, double* var) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(blocks && a, "joint_solve: null pointer");
+    if (ns == 0) return;
+    joint_solve_device(ctx, nobs, ns, K, blocks, a, resid, nrhs, rhs, mean, x, exact_var, var);
+  });
+}
+
 int svh_batch_data(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze) {
   return guarded(ctx, [&] {
     SVH_REQUIRE(Xe && ye, "batch_data: null pointer");

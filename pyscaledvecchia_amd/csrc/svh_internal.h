@@ -213,6 +213,10 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
                          int need_grad, double var_penalty, double log_var_target,
                          double* ll, double* grad, double* fisher, double* beta,
                          int* jitter_power_max);
+// joint.hip
+void joint_solve_device(svh_ctx* ctx, int64_t nobs, int64_t ns, int K, const int64_t* block,
+                        const double* av, const double* resid, int nrhs, const double* rhs,
+                        double* mean, double* x, int exact_var, double* var);
 // mlp.hip
 void mlp_release(svh_ctx* ctx);
 // tune.hip: the candidate to launch at a tunable site.  Returns the cached

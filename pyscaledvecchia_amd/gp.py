@@ -19,8 +19,6 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
-from scipy import sparse
-from scipy.sparse.linalg import spsolve_triangular
 
 from . import _lib
 from .likelihood import DeviceRebuild, VecchiaStructure
@@ -373,9 +371,9 @@ class ScaledVecchiaGP:
         mask[:, :K - 1] = (block[:, :K - 1] < n).astype(float)
         av = self._ctx().block_columns(Xo, block, mask, self.theta_, self.nu_,
                                        precision="fp64")
-        Ufull = sparse.coo_matrix((av.ravel(), (block.ravel(), np.repeat(np.arange(ns), K))),
-                                  shape=(n + ns, ns)).tocsc()
-        return Ufull[:n, :], Ufull[n:, :], order[n:] - n, order_obs
+        # U's column i holds av[i] at rows block[i]; the solves with
+        # Lt = U_pp^T run on the device (svh_joint_solve)
+        return block, av, order[n:] - n, order_obs
 
     def predict_joint(self, Xstar, m=None, n_sim=0, exact_var=False, random_state=None):
         """Joint Vecchia predictive distribution (gp.py:360-402)."""
@@ -384,24 +382,24 @@ class ScaledVecchiaGP:
         Xs = (Xstar - self._lo) / self._span
         m = self.m_pred if m is None else m
         ns = Xs.shape[0]
-        U_op, U_pp, perm, order_obs = self._joint_factor(Xs, m)
+        block, av, perm, order_obs = self._joint_factor(Xs, m)
+        n = self.X_.shape[0]
         inv = np.empty(ns, dtype=np.int64)
         inv[perm] = np.arange(ns)
         Z_tr = self._design(self.X_)
         resid = self.y_ - (Z_tr @ self.beta_ if self.beta_.size else 0.0)
-        Lt = U_pp.T.tocsr()
-        rhs = -(U_op.T @ resid[order_obs])
-        mu = spsolve_triangular(Lt, rhs, lower=True)[inv]
+        ctx = self._ctx()
+        mu, _, var = ctx.joint_solve(n, block, av, resid=resid[order_obs], exact_var=exact_var)
+        mu = mu[inv]
         if self.beta_.size:
             mu = mu + self._design(Xs) @ self.beta_
         out = {"mean": self._ymu + self._ysd * mu}
         rng = np.random.default_rng(self.random_state if random_state is None else random_state)
         if exact_var:
-            S = spsolve_triangular(Lt, np.eye(ns), lower=True)
-            out["var"] = (self._ysd ** 2) * self.b_ * (S ** 2).sum(1)[inv]
+            out["var"] = (self._ysd ** 2) * self.b_ * var[inv]
         if n_sim > 0:
             eps = rng.standard_normal((ns, n_sim))
-            dr = spsolve_triangular(Lt, eps, lower=True)[inv]
+            dr = ctx.joint_solve(n, block, av, rhs=eps)[1][inv]
             samples = (mu[:, None] + math.sqrt(self.b_) * dr).T
             out["samples"] = self._ymu + self._ysd * samples
             if "var" not in out:

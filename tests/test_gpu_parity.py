@@ -428,3 +428,59 @@ def test_device_rebuild_fit_matches_reference_path(gpu):
     assert g1.loglik_ == g2.loglik_
     np.testing.assert_array_equal(g1.theta_, g2.theta_)
     assert g1.n_rebuilds_ == g2.n_rebuilds_
+
+
+@pytest.mark.parametrize("n,ns,m,seed", [(300, 40, 30, 1), (2000, 1500, 40, 2), (500, 1, 20, 3)])
+def test_joint_solve_matches_sparse_triangular(gpu, n, ns, m, seed):
+    """svh_joint_solve (level-scheduled on the device) against
+    scipy.sparse.linalg.spsolve_triangular -- the reference's own call,
+    gp.py:380-401 -- on the same factor: mean, several right-hand sides and
+    the exact variances (row sums of squares of Lt^-1; ns = 1500 spans two
+    identity chunks)."""
+    from scipy import sparse
+    from scipy.sparse.linalg import spsolve_triangular
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    rng = np.random.default_rng(seed)
+    X = rng.random((n, 3))
+    y = np.sin(4 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.01 * rng.standard_normal(n)
+    gp = ScaledVecchiaGP(m_est=15, m_pred=m, var_correction=False).fit(X, y)
+    Xs = rng.random((ns, 3))
+    block, av, perm, order_obs = gp._joint_factor(Xs, m)
+    nobs, K = gp.X_.shape[0], block.shape[1]
+    resid = gp.y_ - gp._design(gp.X_) @ gp.beta_
+    ro = resid[order_obs]
+    U = sparse.coo_matrix((av.ravel(), (block.ravel(), np.repeat(np.arange(ns), K))),
+                          shape=(nobs + ns, ns)).tocsc()
+    Lt = U[nobs:, :].T.tocsr()
+    ref_mu = spsolve_triangular(Lt, -(U[:nobs, :].T @ ro), lower=True)
+    eps = rng.standard_normal((ns, 7))
+    ref_x = spsolve_triangular(Lt, eps, lower=True)
+    S = spsolve_triangular(Lt, np.eye(ns), lower=True)
+    ref_var = (S ** 2).sum(1)
+    mu, x, var = gpu.joint_solve(nobs, block, av, resid=ro, rhs=eps, exact_var=True)
+    np.testing.assert_allclose(mu, ref_mu, rtol=1e-9, atol=1e-12 * np.abs(ref_mu).max())
+    np.testing.assert_allclose(x, ref_x.reshape(ns, -1), rtol=1e-9,
+                               atol=1e-12 * np.abs(ref_x).max())
+    np.testing.assert_allclose(var, ref_var, rtol=1e-9)
+
+
+def test_predict_joint_matches_oracle(gpu):
+    """The whole joint prediction (mean, exact variance, sample paths with
+    the same numpy RNG draws) against the oracle's predict_joint from the same
+    fitted state."""
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    X, y = _sine(400, seed=21)
+    y = y + 0.05 * np.random.default_rng(23).standard_normal(len(y))
+    gp = ScaledVecchiaGP(m_est=15, m_pred=30, n_est=400, trend="linear").fit(X, y)
+    og = O.OracleGP(m_est=15, m_pred=30, n_est=400, trend="linear")
+    for a in ("X_", "y_", "theta_", "beta_", "b_", "_lo", "_span", "_ymu", "_ysd", "d_"):
+        setattr(og, a, getattr(gp, a))
+    Xt = np.random.default_rng(22).random((60, 2))
+    got = gp.predict_joint(Xt, m=30, exact_var=True, n_sim=9, random_state=5)
+    ref = og.predict_joint(Xt, m=30, exact_var=True, n_sim=9, random_state=5)
+    for k in ("mean", "samples"):
+        np.testing.assert_allclose(got[k], ref[k], rtol=1e-8, atol=1e-10 * np.abs(ref[k]).max())
+    # variances: relative to the prior variance of the fitted process (tiny
+    # predictive variances are rounding-level differences of the factors)
+    prior = gp._ysd ** 2 * math.exp(gp.theta_[0]) * gp.b_
+    np.testing.assert_allclose(got["var"], ref["var"], rtol=1e-7, atol=1e-10 * prior)
