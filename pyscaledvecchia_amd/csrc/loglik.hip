@@ -796,7 +796,7 @@ __host__ __device__ inline int rows_lds_doubles(int dm) {
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
-__device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double* lds,
+__device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* lds,
                                           double (&accF)[4], double (&accX)[3],
                                           unsigned long long (&ph)[8]) {
   unsigned long long ph_t = 0;
@@ -812,7 +812,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
   double* xs = hb + 96;                               // coordinates of this half's block
   double* misc = lds + 2 * rows_half_doubles(DM);     // per half: [8h] fail, [8h+1..2] pivot
   double* Mt = misc + 16;                             // 24 x 33 product operands
-  const int64_t b = 2 * pair + h;
+  const int64_t b = b0 + h;   // first block of the pair: b0
   const bool bvalid = b < a.nblk;
   const int K = bvalid ? a.kcnt[b] + 1 : 1;
   const int ch = bvalid ? a.chunk[b] : 0;
@@ -972,7 +972,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t pair, double*
   const int nX = q * q + q;
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
-    const int64_t bh = 2 * pair + hh;
+    const int64_t bh = b0 + hh;
     if (bh >= a.nblk) break;
     const int Kh = a.kcnt[bh] + 1, km = Kh - 1;
     const int chh = a.chunk[bh];
@@ -1071,7 +1071,7 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
   const int64_t npair = (a.nblk + 1) / 2;
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t pp = blockIdx.x; pp < npair; pp += gridDim.x)
-    rows_pair<NU, MIX, DM, PROF>(a, pp, lds, accF, accX, ph);
+    rows_pair<NU, MIX, DM, PROF>(a, 2 * pp, lds, accF, accX, ph);
   if (PROF && lane == 0)
     for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   const int NACC = grad ? P * P + q * q + q : q * q + q;
@@ -1400,6 +1400,56 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
   a.prof = nullptr;
   for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
     split_block<-1, false, false>(a, b, lds, accF, accX, ph);
+  const int P = D.P, q = D.q;
+  const int nX = q * q + q;
+  double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
+  write_fisher<false>(part, accF, P, lane);
+  for (int e = lane; e < nX; e += kWave)
+    part[P * P + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
+}
+
+// The rows-in-lanes pass 1 for the batch: dataset i's blocks in pairs (the
+// pairs start at the dataset's first block), nu at run time, coordinates in DM
+// >= every dataset's d registers.
+template <int DM>
+__global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
+  extern __shared__ double lds[];
+  const int i = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+  if (!fs_evaluates(B.fs[i].status)) return;
+  const DSDesc D = B.ds[i];
+  const int64_t npair = (D.nblk + 1) / 2;
+  // the partial sums read min(nblk, GX) workgroups per dataset: those past the
+  // last pair still write (zero) partials
+  if (x >= D.nblk) return;
+  LLArgs a;
+  a.X = B.X + D.xoff;
+  a.y = B.y + D.yoff;
+  a.Z = B.Z + D.zoff;
+  a.rows = B.rows;
+  a.nbr = B.nbr;
+  a.kcnt = B.kcnt;
+  a.chunk = B.chunk;
+  a.level = B.level;
+  a.fail = B.fail;
+  a.inv_lam = B.par + i * 16 + 2;
+  a.cache = B.cache;
+  a.partial = nullptr;
+  a.nblk = D.blk0 + D.nblk;
+  a.d = D.d;
+  a.q = D.q;
+  a.mmax = B.mmax_stride;
+  a.P = D.P;
+  a.need_grad = 1;
+  a.nu_code = D.nu_code;
+  a.var = B.par[i * 16 + 0];
+  a.nug = B.par[i * 16 + 1];
+  a.rec_stride = kBNC;
+  a.prof = nullptr;
+  double accF[4] = {0.0, 0.0, 0.0, 0.0};
+  double accX[3] = {0.0, 0.0, 0.0};
+  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t pp = x; pp < npair; pp += B.GX)
+    rows_pair<-1, false, DM, false>(a, D.blk0 + 2 * pp, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
   const int nX = q * q + q;
   double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
@@ -2350,8 +2400,21 @@ static void batch_enqueue(svh_ctx* ctx, bool with_update) {
   a.partial = partial;
   a.mmax_stride = B.mmax;
   a.GX = B.GX;
-  const size_t lds = (size_t)split_lds_doubles(B.dmax) * sizeof(double);
-  hipLaunchKernelGGL(ll_pass1_batch, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+  // pass 1: the rows-in-lanes kernel (SVH_BATCH_KERNEL=split: the split-row one)
+  const char* bk = std::getenv("SVH_BATCH_KERNEL");
+  if (bk && std::string(bk) == "split") {
+    const size_t lds = (size_t)split_lds_doubles(B.dmax) * sizeof(double);
+    hipLaunchKernelGGL(ll_pass1_batch, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+  } else {
+    const int dm = B.dmax <= 4 ? 4 : (B.dmax <= 8 ? 8 : 10);
+    const size_t lds = (size_t)rows_lds_doubles(dm) * sizeof(double);
+    if (dm == 4)
+      hipLaunchKernelGGL(ll_pass1_batch_rows<4>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+    else if (dm == 8)
+      hipLaunchKernelGGL(ll_pass1_batch_rows<8>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+    else
+      hipLaunchKernelGGL(ll_pass1_batch_rows<10>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
+  }
   hipLaunchKernelGGL(ll_sum_batch, dim3(kBNACC, B.ndata), dim3(256), 0, s, dd, fs, partial, B.GX,
                      tot);
   hipLaunchKernelGGL(ll_beta_batch, dim3(B.ndata), dim3(256), 0, s, dd, fs, tot, beta, out);
