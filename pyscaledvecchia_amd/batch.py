@@ -16,7 +16,9 @@ Limits of the batched kernels: d <= 10, trend columns <= 11, m_est <= 31.
 from __future__ import annotations
 
 import math
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -198,14 +200,20 @@ class BatchedFit:
         """Device rebuild of datasets `which` at `thetas` (maximin order and
         ordered neighbours of the scaled inputs, gp.py:182-191), the first
         point chosen on the host as ordering.py:72 does."""
-        lams, firsts, ms = [], [], []
-        for i, th in zip(which, thetas):
-            p = preps[i]
-            lam = np.exp(np.asarray(th, float)[1:1 + p["d"]])
-            Xs = p["Xe"] / lam
-            firsts.append(int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1))))
-            lams.append(lam)
-            ms.append(p["m"])
+        lams = [np.exp(np.asarray(th, float)[1:1 + preps[i]["d"]])
+                for i, th in zip(which, thetas)]
+        ms = [preps[i]["m"] for i in which]
+
+        def first(k):
+            Xs = preps[which[k]]["Xe"] / lams[k]
+            return int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1)))   # ordering.py:72
+        if len(which) > 4:
+            # NumPy releases the GIL in these array passes: one host thread per
+            # dataset group
+            with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+                firsts = list(ex.map(first, range(len(which))))
+        else:
+            firsts = [first(k) for k in range(len(which))]
         bh.rebuild(which, lams, firsts, ms)
 
     @staticmethod

@@ -176,15 +176,22 @@ def _spectral(rng, nu, d, F):
     return z * np.sqrt(2 * nu / u), rng.uniform(0, 2 * math.pi, F)
 
 
-def simulate_prior(ctx, count, n, seed, d_range=(2, 10), F=1024):
+def simulate_prior(ctx, count, n, seed, d_range=(2, 10), F=1024, p_inert=0.0):
     """Datasets from the prior (X float32 storage, Matern nu draws by random
-    Fourier features plus nugget noise).  Returns [(X, y, nu, truth)]."""
+    Fourier features plus nugget noise).  With p_inert > 0 each input is, with
+    that probability, inert (range 1e4: the draw is flat along it; at least
+    one input stays active).  Returns [(X, y, nu, truth)]."""
     rng = np.random.default_rng(seed)
     specs, Xs, Ws, bs, amps = [], [], [], [], []
     for _ in range(count):
         d = int(rng.integers(d_range[0], d_range[1] + 1))
         nu = float(rng.choice([0.5, 1.5, 2.5]))
         lam = 10.0 ** rng.uniform(-1, 1.5, d)
+        if p_inert > 0.0:
+            inert = rng.random(d) < p_inert
+            if inert.all():
+                inert[int(rng.integers(d))] = False
+            lam = np.where(inert, 1e4, lam)
         tau = math.exp(rng.uniform(math.log(1e-8), math.log(1e-2)))
         X = rng.random((n, d)).astype(np.float32).astype(np.float64)
         W, b = _spectral(rng, nu, d, F)
@@ -205,7 +212,7 @@ def simulate_prior(ctx, count, n, seed, d_range=(2, 10), F=1024):
 
 
 def build_training_set(count, n=5000, seed=1, batch=256, verbose=True, device=0,
-                       checkpoint=None):
+                       checkpoint=None, p_inert=0.0):
     """Simulate `count` datasets, fit each with the batched fitter (default
     start, m_est = 30, constant trend), and return features, targets, masks.
     `checkpoint`: a path rewritten with the rows so far after each batch."""
@@ -215,7 +222,7 @@ def build_training_set(count, n=5000, seed=1, batch=256, verbose=True, device=0,
     F, T, M, info = [], [], [], []
     t0 = time.perf_counter()
     for a in range(0, count, batch):
-        data = simulate_prior(ctx, min(batch, count - a), n, seed + a)
+        data = simulate_prior(ctx, min(batch, count - a), n, seed + a, p_inert=p_inert)
         models = [ScaledVecchiaGP(nu=nu, m_est=30, n_est=n, trend="constant",
                                   var_correction=False) for _, _, nu, _ in data]
         try:
@@ -357,6 +364,8 @@ def main(argv=None):
     g.add_argument("--n", type=int, default=5000)
     g.add_argument("--seed", type=int, default=1)
     g.add_argument("--batch", type=int, default=256)
+    g.add_argument("--inert", type=float, default=0.0,
+                   help="probability that an input of a simulated dataset is inert")
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True, nargs="+")
@@ -373,7 +382,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "generate":
         t0 = time.perf_counter()
-        F, T, M, info = build_training_set(a.datasets, a.n, a.seed, a.batch, checkpoint=a.out)
+        F, T, M, info = build_training_set(a.datasets, a.n, a.seed, a.batch, checkpoint=a.out,
+                                           p_inert=a.inert)
         np.savez(a.out, F=F, T=T, M=M, info=info)
         print(json.dumps({"datasets": len(F), "seconds": time.perf_counter() - t0}))
     elif a.cmd == "train":
