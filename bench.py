@@ -314,8 +314,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5p", "c5j"])
-    ap.add_argument("--n-train", type=int, default=50_000)
-    ap.add_argument("--n-test", type=int, default=20_000)
+    ap.add_argument("--n-train", type=int, default=200_000)
+    ap.add_argument("--n-test", type=int, default=100_000)
     ap.add_argument("--n-c4", type=int, default=50_000)
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--fit-only", action="store_true",
@@ -334,8 +334,8 @@ def main():
         run_c3(args)
         return
     if args.workload == "c5j":
-        if args.n_test == 20_000:
-            args.n_test = 10_000
+        if args.n_test == 100_000:
+            args.n_test = 10_000     # the sampled path
         run_c5j(args)
         return
     if args.workload == "c5p":
