@@ -782,6 +782,16 @@ void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mi
 //     L^-1 Z] on the matrix cores, block by block (as in the split kernel).
 constexpr int kRQ = 11, kRC = 12;
 
+// The rows kernels run one wave per workgroup, and a wave's LDS operations
+// complete in issue order: exchanging values between its lanes through LDS
+// needs only a compiler fence, not a workgroup barrier with its wait for every
+// outstanding LDS operation.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ double half_sum(double v) {
 #pragma unroll
   for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
@@ -837,7 +847,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     for (int t = 0; t < K; ++t) tr += var * (1.0 + nug);
     jit = kJitDev[lev] * (tr / K);
   }
-  __syncthreads();
+  wave_sync();
   SVH_PHASE(0)
   // ---- Sigma row r, lower part (upper part zero)
   double S[32];
@@ -871,7 +881,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
       for (int c = 0; c < 1 + kRQ; ++c) bb[c] = bv[c];
     }
-    __syncthreads();
+    wave_sync();
     const double pk = cb[k];
     fail |= (k < K) && !(pk > 0.0);
     const double ik = rcp_nr(pk);
@@ -890,7 +900,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     for (int c = k + 1; c < 32; ++c) asm volatile("" : "+v"(S[c]));
 #pragma unroll
     for (int c = 0; c < 1 + kRQ; ++c) asm volatile("" : "+v"(bv[c]));
-    __syncthreads();
+    wave_sync();
     __builtin_amdgcn_sched_barrier(0);
   }
   const double dinv = rsqrt_nr(pr);   // 1 / L_rr
@@ -902,7 +912,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     misc[8 * h + 2] = dinv;
   }
   if (r == 0) misc[8 * h] = fail ? 1.0 : 0.0;
-  __syncthreads();
+  wave_sync();
   const double pK = misc[8 * h + 1], dK = misc[8 * h + 2];
   SVH_PHASE(2)
   double R[kRC];
@@ -919,7 +929,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     }
     const double av = t * dK;
     ab[r] = av;
-    __syncthreads();
+    wave_sync();
     SVH_PHASE(3)
     // ---- dS_p a, row r
     if (act) {
@@ -952,7 +962,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
         for (int c = 0; c < kRC; ++c) wb[c] = R[c];
       }
-      __syncthreads();
+      wave_sync();
       if (r > k) {
         const double l = S[k];
 #pragma unroll
@@ -960,7 +970,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       }
 #pragma unroll
       for (int c = 0; c < kRC; ++c) asm volatile("" : "+v"(R[c]));
-      __syncthreads();
+      wave_sync();
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
@@ -976,7 +986,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     if (bh >= a.nblk) break;
     const int Kh = a.kcnt[bh] + 1, km = Kh - 1;
     const int chh = a.chunk[bh];
-    __syncthreads();
+    wave_sync();
     if (h == hh) {
       if (grad) {
         Mt[0 * 33 + r] = R[0];
@@ -990,7 +1000,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       for (int c = 0; c < kRQ; ++c)
         if (c < q) Mt[(cyl + 1 + c) * 33 + r] = bv[1 + c];
     }
-    __syncthreads();
+    wave_sync();
     const bool okh = misc[8 * hh] == 0.0;
     if (!okh) {
       if (lane == 0) atomicOr(a.fail + chh, 1);

@@ -484,3 +484,13 @@ def test_predict_joint_matches_oracle(gpu):
     # predictive variances are rounding-level differences of the factors)
     prior = gp._ysd ** 2 * math.exp(gp.theta_[0]) * gp.b_
     np.testing.assert_allclose(got["var"], ref["var"], rtol=1e-7, atol=1e-10 * prior)
+
+
+def test_nu_property_matches_fixed_setting(gpu):
+    """tests/test_gp.py:185-190 of the reference: nu is not estimated and nu_
+    reports the fixed value."""
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    gp = ScaledVecchiaGP(m_est=8, m_pred=15, n_est=100, nu=2.5,
+                         random_state=0).fit(*_sine(100, seed=23))
+    assert gp._estimate_nu is False
+    assert gp.nu_ == 2.5
