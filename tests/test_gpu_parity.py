@@ -410,19 +410,24 @@ def test_device_rebuild_matches_host_structure(gpu):
         assert dev.structure(th, nu, need_grad=False)[0] == host(th, nu, need_grad=False)[0]
 
 
-def test_device_rebuild_fit_matches_reference_path(gpu):
+@pytest.mark.parametrize("n,d,m,trend", [(2500, 4, 15, "constant"), (400, 12, 10, "zero"),
+                                         (60, 3, 59, "linear")])
+def test_device_rebuild_fit_matches_reference_path(gpu, n, d, m, trend):
     """A whole fit with device rebuilds equals the fit through the host
-    rebuild path (same theta, same log-likelihood, bit for bit)."""
+    rebuild path (same theta, same log-likelihood, bit for bit): the C2-like
+    case, d = 12 (widest maximin registers; the likelihood falls back from the
+    rows kernel), and m = n - 1 (every earlier point a neighbour)."""
     from pyscaledvecchia_amd import ScaledVecchiaGP
     from pyscaledvecchia_amd import likelihood as LK
     rng = np.random.default_rng(5)
-    X = rng.random((2500, 4))
-    y = np.sin(5 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.05 * rng.standard_normal(2500)
-    g1 = ScaledVecchiaGP(m_est=15, var_correction=False).fit(X, y)
+    X = rng.random((n, d))
+    y = np.sin(5 * X[:, 0]) + X[:, 1] * X[:, 2] + 0.05 * rng.standard_normal(n)
+    kw = dict(m_est=m, n_est=n, trend=trend, var_correction=False)
+    g1 = ScaledVecchiaGP(**kw).fit(X, y)
     keep = LK.DeviceRebuild.supported
     try:
         LK.DeviceRebuild.supported = staticmethod(lambda d, m: False)
-        g2 = ScaledVecchiaGP(m_est=15, var_correction=False).fit(X, y)
+        g2 = ScaledVecchiaGP(**kw).fit(X, y)
     finally:
         LK.DeviceRebuild.supported = keep
     assert g1.loglik_ == g2.loglik_
