@@ -2263,8 +2263,12 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   B.nchunk = (int)co;
   int64_t maxn = 0;
   for (auto& D : B.desc) maxn = std::max(maxn, D.nblk);
-  B.GX = (int)std::min<int64_t>(maxn, 1024);
-  B.G2X = (int)std::min<int64_t>((maxn + 255) / 256, 64);
+  // workgroups per dataset: at least 1024 (pass 1) / 64 (pass 2), and with
+  // few datasets enough for two waves per SIMD of the chip (8 one-wave
+  // workgroups per CU in pass 1, as the single-dataset launch uses)
+  const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
+  B.GX = (int)std::min<int64_t>(maxn, std::max(1024, (8 * ncu + ndata - 1) / ndata));
+  B.G2X = (int)std::min<int64_t>((maxn + 255) / 256, std::max(64, (2 * ncu + ndata - 1) / ndata));
   // block tables (rows are 0..n-1 within each dataset, in order)
   std::vector<int> hrows(bo), hnbr((size_t)bo * mmax, -1), hk(bo), hc(bo), chunk_ds(co);
   for (int i = 0; i < ndata; ++i) {

@@ -1174,8 +1174,8 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   if (mprof) {
     unsigned long long h[8];
     SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));
-    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu sort %llu window %llu "
-                 "update %llu\n", rounds, h[4], h[0], h[1], h[2], h[3]);
+    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu window %llu update %llu "
+                 "resort %llu\n", rounds, h[4], h[0], h[1], h[2], h[3]);
   }
 }
 

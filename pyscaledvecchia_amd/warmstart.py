@@ -18,8 +18,9 @@ RNG, so the fit's RNG stream is untouched):
 Targets: the fitted log-parameters, padded to 14 = [sigma^2, lambda_1..12,
 tau] with a mask of the d + 2 valid entries.  Features and targets are
 normalised by their training means and standard deviations.  The starting
-point takes sigma^2 and the ranges from the network and the default nugget
-(see unpack_theta).
+point takes sigma^2 from the network, log-ranges halfway between the
+network's and the default start's, and the default nugget (see
+unpack_theta).
 
 Network: 64 -> 3 x (256, GELU) -> 14, bf16 weights with float32
 accumulation; forward, backward and Adam are HIP kernels (svh_mlp_*).
@@ -107,13 +108,25 @@ LOG_NUG0 = math.log(0.1)
 
 
 def unpack_theta(out, d):
-    """Starting point from a network output: sigma^2 and the ranges from the
-    network (ranges clipped to [0.05, 100]), the nugget from the default start
-    (log 0.1).  The nugget is barely predictable from the features (held-out
-    correlation ~0.2) and small predicted nuggets sit where the likelihood is
-    flat in tau, where Fisher scoring's stopping rule can end the search
-    after one short step."""
-    lam = np.clip(out[1:1 + d], LOG_LAM_LO, LOG_LAM_HI)
+    """Starting point from a network output: sigma^2 from the network, each
+    log-range halfway between the network's and the default start's
+    log(0.2 sqrt d) (then clipped to [0.05, 100]), the nugget from the default
+    start (log 0.1).
+
+    The nugget is barely predictable from the features (held-out correlation
+    ~0.2) and small predicted nuggets sit where the likelihood is flat in tau,
+    where Fisher scoring's stopping rule can end the search after one short
+    step.  The ranges are damped because the objective has many stationary
+    points (it changes with every reordering) and a range far beyond the unit
+    box switches its input off, where the likelihood is flat in that range:
+    with the network's ranges as they are, a sin-sum function of 8 inputs
+    (n = 50,000, nugget ~3e-6) ends at half the default start's
+    log-likelihood with one weak input switched off; halfway, it ends above
+    it.  On 256 held-out prior draws (n = 50,000) the halfway start saves 1.25
+    of 15.3 iterations (the undamped one 1.34), ends no worse than the
+    default start for 64 % of them (61 %) and at worst 0.18 below in
+    relative log-likelihood (0.35)."""
+    lam = np.clip(0.5 * (out[1:1 + d] + math.log(0.2 * math.sqrt(d))), LOG_LAM_LO, LOG_LAM_HI)
     return np.concatenate([[out[0]], lam, [LOG_NUG0]]).astype(float)
 
 
