@@ -60,6 +60,13 @@ int svh_maximin(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int64_t fir
 int svh_ordered_knn(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t m,
                     int64_t start, int64_t* nn);
 
+/* Rows start..n-1 of svh_ordered_knn only: writes nn ((n - start) x (m+1)),
+ * its row i - start being row i of the full result.  The form gp.py:326-333
+ * reads (find_ordered_nn with start = n, ordering.py:76-116, of which only
+ * the prediction rows are used), without the n x (m+1) rows before start. */
+int svh_ordered_knn_rows(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t m,
+                         int64_t start, int64_t* nn);
+
 /* k nearest rows of Xref (nref x d) for each row of Xq (nq x d), sorted by
  * (squared distance, index).  Writes nb (nq x k).
  * Replaces the cKDTree query of gp.py:253-254. */

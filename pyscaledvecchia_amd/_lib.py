@@ -47,6 +47,8 @@ def _load():
     L.svh_maximin.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, _i64p]
     L.svh_ordered_knn.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_int64, _i64p]
+    L.svh_ordered_knn_rows.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int64, _i64p]
     L.svh_knn_query.argtypes = [_vp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32,
                                 ctypes.c_int32, _i64p]
     L.svh_vecchia_setup.argtypes = [_vp, _dp, _dp, _dp, ctypes.c_int64, ctypes.c_int32,
@@ -105,8 +107,8 @@ def _load():
     for name in ("svh_ordering_batch", "svh_batch_setup", "svh_batch_update", "svh_batch_data",
                  "svh_batch_rebuild", "svh_joint_solve", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
-                 "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_knn_query", "svh_vecchia_setup",
-                 "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
+                 "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_ordered_knn_rows", "svh_knn_query",
+                 "svh_vecchia_setup", "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -121,7 +123,7 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
             "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild",
-            "svh_joint_solve")
+            "svh_joint_solve", "svh_ordered_knn_rows")
 
 
 def device_count() -> int:
@@ -224,6 +226,15 @@ class Context:
         nn = np.empty((n, m + 1), dtype=np.int64)
         self._check(self._L.svh_ordered_knn(self._h, _ptr(X), n, d, int(m), int(start),
                                             _ptr(nn, _i64p)))
+        return nn
+
+    def ordered_knn_rows(self, X, m, start):
+        """Rows start..n-1 of ordered_knn(X, m, start)."""
+        X = _finite(X, "X")
+        n, d = X.shape
+        nn = np.empty((n - int(start), m + 1), dtype=np.int64)
+        self._check(self._L.svh_ordered_knn_rows(self._h, _ptr(X), n, d, int(m), int(start),
+                                                 _ptr(nn, _i64p)))
         return nn
 
     def knn_query(self, Xref, Xq, k):

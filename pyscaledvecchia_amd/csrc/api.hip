@@ -157,6 +157,26 @@ int svh_ordered_knn(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t
   });
 }
 
+int svh_ordered_knn_rows(svh_ctx* ctx, const double* X, int64_t n, int32_t d, int32_t m,
+                         int64_t start, int64_t* nn) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(n >= 0 && d >= 1 && d <= 32 && m >= 0, "ordered_knn_rows: bad n/d/m");
+    SVH_REQUIRE(start >= 0 && start <= n, "ordered_knn_rows: need 0 <= start <= n");
+    if (n == start) return;
+    SVH_REQUIRE(n < (1ll << 31), "ordered_knn_rows: n too large");
+    SVH_REQUIRE(X && nn, "ordered_knn_rows: null pointer");
+    double* dX = ctx->arena.get_t<double>("nn.X", (size_t)n * d);
+    double* dXT = ctx->arena.get_t<double>("nn.XT", (size_t)n * d);
+    int64_t* dnn = ctx->arena.get_t<int64_t>("nn.out", (size_t)(n - start) * (m + 1));
+    upload(ctx, dX, X, (size_t)n * d * sizeof(double));
+    launch_to_soa(ctx, dX, n, d, nullptr, dXT);
+    launch_ordered_knn(ctx, dXT, n, d, m, start, dnn, start);
+    SVH_HIP(hipGetLastError());
+    download(ctx, nn, dnn, (size_t)(n - start) * (m + 1) * sizeof(int64_t));
+    sync(ctx);
+  });
+}
+
 int svh_knn_query(svh_ctx* ctx, const double* Xref, int64_t nref, const double* Xq, int64_t nq,
                   int32_t d, int32_t k, int64_t* nb) {
   return guarded(ctx, [&] {

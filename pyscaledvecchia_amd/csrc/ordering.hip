@@ -758,9 +758,10 @@ __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __
 struct KNNDesc {
   const double* XT;   // reference coordinates, SoA d x nref
   const double* QT;   // query coordinates, SoA d x nq (query form)
-  int64_t* out;
+  int64_t* out;       // row q at out + (q - row0) * out_stride
   int64_t nref, nq, rowstart;
   int d, m, ordered, out_stride;
+  int64_t row0;       // ordered form: rows below row0 are not written
 };
 
 template <int LCAP>
@@ -778,11 +779,12 @@ __global__ __launch_bounds__(kWave) void knn_kernel(const KNNDesc* __restrict__ 
   const int lane = threadIdx.x;
   int64_t hi;
   int k;
-  int64_t* orow = out + q * (int64_t)out_stride;
+  int64_t* orow = out + (q - D.row0) * (int64_t)out_stride;
   if (ordered) {
     hi = q;
     k = (int)((q < (int64_t)m) ? q : (int64_t)m);
     if (q < rowstart) k = 0;
+    if (q < D.row0) return;
     for (int t = lane; t <= m; t += kWave) orow[t] = (t == 0) ? q : -1;
   } else {
     hi = nref;
@@ -901,13 +903,14 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
   const bool qvalid = q < nqry;
   int64_t hi = 0;
   int k = 0;
-  int64_t* orow = D.out + q * (int64_t)out_stride;
+  int64_t* orow = D.out + (q - D.row0) * (int64_t)out_stride;
   if (qvalid) {
     if (ordered) {
       hi = q;
       k = (int)((q < (int64_t)m) ? q : (int64_t)m);
       if (q < rowstart) k = 0;
-      for (int t = lane; t <= m; t += kWave) orow[t] = (t == 0) ? q : -1;
+      if (q >= D.row0)
+        for (int t = lane; t <= m; t += kWave) orow[t] = (t == 0) ? q : -1;
     } else {
       hi = nref;
       k = m;
@@ -915,6 +918,9 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
   }
   // candidates of the workgroup: up to the last query's bound
   const int64_t qlast = (q0 + kKnnWaves - 1 < nqry ? q0 + kKnnWaves - 1 : nqry - 1);
+  // ordered rows before rowstart have no neighbours to find: a workgroup of
+  // only such rows loads no tiles (the whole workgroup returns)
+  if (ordered && qlast < rowstart) return;
   const int64_t hwg = ordered ? qlast : nref;
   double qx[DM];
   const double* qcol = ordered ? XT : D.QT;
@@ -1195,14 +1201,15 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
 }
 
 void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m, int64_t start,
-                        int64_t* dnn) {
-  KNNDesc D{dXT, nullptr, dnn, n, 0, start < 1 ? 1 : start, d, m, 1, m + 1};
+                        int64_t* dnn, int64_t row0) {
+  SVH_REQUIRE(row0 >= 0 && row0 <= start, "ordered_knn: first output row beyond start");
+  KNNDesc D{dXT, nullptr, dnn, n, 0, start < 1 ? 1 : start, d, m, 1, m + 1, row0};
   launch_knn_descs(ctx, std::vector<KNNDesc>{D});
 }
 
 void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const double* dXqT,
                       int64_t nq, int d, int k, int64_t* dnb, int nb_stride) {
-  KNNDesc D{dXrefT, dXqT, dnb, nref, nq, 0, d, k, 0, nb_stride};
+  KNNDesc D{dXrefT, dXqT, dnb, nref, nq, 0, d, k, 0, nb_stride, 0};
   launch_knn_descs(ctx, std::vector<KNNDesc>{D});
 }
 
@@ -1257,7 +1264,7 @@ void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t
   for (int i = 0; i < nd; ++i) {
     launch_to_soa(ctx, dX + hm[i].xoff, n[i], d[i], nullptr, dXoT + hm[i].xoff);
     hk[i] = KNNDesc{dXoT + hm[i].xoff, nullptr, dnn + hm[i].soff * (m + 1), n[i], 0, 1, d[i], m, 1,
-                    m + 1};
+                    m + 1, 0};
   }
   launch_knn_descs(ctx, hk);
   SVH_HIP(hipMemcpyAsync(nn, dnn, (size_t)so * (m + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -1354,7 +1361,7 @@ void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const d
     launch_gather_soa(ctx, dXT + hm[k].xoff, dord + hm[k].soff, lay[k].n, lay[k].d,
                       dXTo + hm[k].xoff);
     hk[k] = KNNDesc{dXTo + hm[k].xoff, nullptr, dnn + hm[k].soff * (mmax + 1), lay[k].n, 0, 1,
-                    lay[k].d, m[k], 1, mmax + 1};
+                    lay[k].d, m[k], 1, mmax + 1, 0};
   }
   launch_knn_descs(ctx, hk);
   double* bX = (double*)ctx->arena.get("b.X", 0);

@@ -186,7 +186,7 @@ struct svh_ctx {
 void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n, int d, int64_t first,
                     int64_t* dorder);
 void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m,
-                        int64_t start, int64_t* dnn);
+                        int64_t start, int64_t* dnn, int64_t row0 = 0);
 void launch_knn_query(svh_ctx* ctx, const double* dXrefT, int64_t nref, const double* dXqT,
                       int64_t nq, int d, int k, int64_t* dnb, int nb_stride);
 void launch_to_soa(svh_ctx* ctx, const double* dX, int64_t n, int d, const double* dlam,

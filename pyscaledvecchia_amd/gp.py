@@ -356,16 +356,24 @@ class ScaledVecchiaGP:
         ranges = self.ranges_
         Xall = np.vstack([self.X_, Xs])
         Xsc = Xall / ranges
-        order_obs = maximin_order(Xsc[:n])
+        # The reference orders the observations by maximin too (gp.py:321), an
+        # O(n^2 d) pass that changes no output: every prediction point's
+        # neighbours are chosen among all observations whatever their order
+        # (sets by (distance, position): only exact distance ties could pick
+        # another neighbour), and the order only permutes U_op's rows together
+        # with resid[order_obs].  The observations keep their input order.
+        order_obs = np.arange(n)
         order_pred = maximin_order(Xsc[n:]) + n
         order = np.concatenate([order_obs, order_pred])
         Xo_sc = Xsc[order]
         mm = min(m, n)
-        nn = find_ordered_nn(Xo_sc, mm, start=n)
+        # rows n.. of find_ordered_nn(Xo_sc, mm, start=n) (gp.py:326): the only
+        # rows read below
+        nn = self._ctx().ordered_knn_rows(Xo_sc, mm, n)
         Xo = Xall[order]
         K = mm + 1
         pred_rows = np.arange(n, n + ns)
-        block = np.concatenate([nn[pred_rows, 1:K], pred_rows[:, None]], axis=1)
+        block = np.concatenate([nn[:, 1:K], pred_rows[:, None]], axis=1)
         mask = np.ones((ns, K))
         mask[:, K - 1] = 0.0
         mask[:, :K - 1] = (block[:, :K - 1] < n).astype(float)
@@ -389,17 +397,20 @@ class ScaledVecchiaGP:
         Z_tr = self._design(self.X_)
         resid = self.y_ - (Z_tr @ self.beta_ if self.beta_.size else 0.0)
         ctx = self._ctx()
-        mu, _, var = ctx.joint_solve(n, block, av, resid=resid[order_obs], exact_var=exact_var)
+        # a fresh generator whose only draw is eps (gp.py:389-396): drawn
+        # first, so the mean, the paths and the exact variances are one solve
+        rng = np.random.default_rng(self.random_state if random_state is None else random_state)
+        eps = rng.standard_normal((ns, n_sim)) if n_sim > 0 else None
+        mu, dr, var = ctx.joint_solve(n, block, av, resid=resid[order_obs], rhs=eps,
+                                      exact_var=exact_var)
         mu = mu[inv]
         if self.beta_.size:
             mu = mu + self._design(Xs) @ self.beta_
         out = {"mean": self._ymu + self._ysd * mu}
-        rng = np.random.default_rng(self.random_state if random_state is None else random_state)
         if exact_var:
             out["var"] = (self._ysd ** 2) * self.b_ * var[inv]
         if n_sim > 0:
-            eps = rng.standard_normal((ns, n_sim))
-            dr = ctx.joint_solve(n, block, av, rhs=eps)[1][inv]
+            dr = dr[inv]
             samples = (mu[:, None] + math.sqrt(self.b_) * dr).T
             out["samples"] = self._ymu + self._ysd * samples
             if "var" not in out:

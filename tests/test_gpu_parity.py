@@ -63,6 +63,16 @@ def test_ordered_knn_bit_exact(gpu, n, d, m, start, seed):
     np.testing.assert_array_equal(gpu.ordered_knn(X, m, start), O.find_ordered_nn(X, m, start))
 
 
+@pytest.mark.parametrize("n,d,m,start", [(3000, 10, 140, 2500), (700, 14, 30, 650),
+                                         (400, 3, 20, 0), (90, 2, 8, 90)])
+def test_ordered_knn_rows_match_full_rows(gpu, n, d, m, start):
+    """svh_ordered_knn_rows = rows start.. of svh_ordered_knn (tiled kernel for
+    d <= 12, one-wave kernel beyond; start = n gives no rows)."""
+    X = np.random.default_rng(n + d).random((n, d))
+    full = gpu.ordered_knn(X, m, start)
+    np.testing.assert_array_equal(gpu.ordered_knn_rows(X, m, start), full[start:])
+
+
 def test_ordered_knn_ties(gpu):
     g = np.arange(12.0)
     X = np.array([[a, b] for a in g for b in g])
@@ -472,7 +482,9 @@ def test_joint_solve_matches_sparse_triangular(gpu, n, ns, m, seed):
 def test_predict_joint_matches_oracle(gpu):
     """The whole joint prediction (mean, exact variance, sample paths with
     the same numpy RNG draws) against the oracle's predict_joint from the same
-    fitted state."""
+    fitted state.  The oracle orders the observations by maximin as
+    gp.py:321 does; the estimator keeps their input order (the order changes
+    no output)."""
     from pyscaledvecchia_amd import ScaledVecchiaGP
     X, y = _sine(400, seed=21)
     y = y + 0.05 * np.random.default_rng(23).standard_normal(len(y))
