@@ -217,34 +217,6 @@ class BatchedFit:
         bh.rebuild(which, lams, firsts, ms)
 
     @staticmethod
-    def _structures(ctx, models, preps, thetas, mmax):
-        """Structures of several datasets, their orderings and neighbour sets
-        computed together (svh_ordering_batch)."""
-        by_m = {}
-        for k, p in enumerate(preps):
-            by_m.setdefault(p["m"], []).append(k)
-        out = [None] * len(preps)
-        for m, ks in by_m.items():
-            Xs = [preps[k]["Xe"] / np.exp(np.asarray(thetas[k])[1:1 + preps[k]["d"]]) for k in ks]
-            res = ctx.ordering_batch(Xs, m)
-            for k, (order, nn) in zip(ks, res):
-                out[k] = BatchedFit._assemble(preps[k], order, nn, mmax)
-        return out
-
-    @staticmethod
-    def _assemble(p, order, nn, mmax):
-        Xe, ye, Ze, m = p["Xe"], p["ye"], p["Ze"], p["m"]
-        groups = _nn_groups(nn, m)
-        n = len(order)
-        kcnt = np.minimum(np.arange(n), m).astype(np.int32)
-        nbr = np.full((n, mmax), -1, np.int64)
-        nbr[:, :m] = nn[:, 1:m + 1]
-        chunk, nchunk = _chunk_ids(groups, p["d"] + 2)
-        return dict(Xo=np.ascontiguousarray(Xe[order]), yo=np.ascontiguousarray(ye[order]),
-                    Zo=np.ascontiguousarray(Ze[order]), nbr=nbr, kcnt=kcnt,
-                    chunk=np.asarray(chunk, np.int32), nchunk=nchunk)
-
-    @staticmethod
     def _structure(g, p, theta, mmax):
         """Ordered inputs, per-row neighbour lists and jitter-chunk ids of one
         dataset at `theta` (gp.py:182-191 + likelihood.py:111-114)."""

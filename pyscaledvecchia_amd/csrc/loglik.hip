@@ -12,7 +12,7 @@
 //             block is cached as a short record.
 //   ll_reduce1 (1 workgroup): fixed-order sum of the partials, beta =
 //             solve(XtX + 1e-12 I, Zty) by partial-pivot elimination.
-//   ll_pass2  (grid-stride, one thread per block): z_K = u - v.beta,
+//   ll_pass2  (grid-stride, 1 + P lanes per block record): z_K = u - v.beta,
 //             W.z = s - T beta, loglik and gradient terms.
 //   ll_reduce2 (1 workgroup): fixed-order sum, failed-chunk count.
 // All reductions are deterministic.
@@ -1217,44 +1217,45 @@ __global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------- pass 2
+// Pass 2: G consecutive lanes per record (G = 1 without the gradient, else
+// the power of two >= 1 + P): lane t of a record's group owns the value term
+// (t = 0) or derivative p = t - 1; every lane of the group forms zK (its loads
+// of the record head hit the same addresses).  Records are read by G lanes in
+// parallel instead of one thread walking 2 + q + 2P + Pq doubles.
 __global__ __launch_bounds__(256) void ll_pass2(const double* __restrict__ cache, int64_t nblk,
-                                                int P, int q, int need_grad,
+                                                int P, int q, int need_grad, int G,
                                                 const double* __restrict__ beta,
                                                 double* __restrict__ partial2) {
   __shared__ double red[256];
   const int NC = need_grad ? 2 + q + 2 * P + P * q : 2 + q;
   const int NV = need_grad ? 1 + P : 1;
-  double bet[32];
-  for (int c = 0; c < q; ++c) bet[c] = beta[c];
-  double accv[40];
-  for (int e = 0; e < NV; ++e) accv[e] = 0.0;
-  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nblk;
-       b += (int64_t)gridDim.x * blockDim.x) {
+  const int t = threadIdx.x & (G - 1);
+  const int64_t rstride = (int64_t)gridDim.x * (blockDim.x / G);
+  double acc = 0.0;
+  for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; b < nblk; b += rstride) {
     const double* rec = cache + b * NC;
     double zK = rec[0];
-    for (int c = 0; c < q; ++c) zK -= rec[1 + c] * bet[c];
-    accv[0] += -0.5 * (2.0 * rec[1 + q] + zK * zK);
-    if (need_grad) {
+    for (int c = 0; c < q; ++c) zK -= rec[1 + c] * beta[c];
+    if (t == 0) {
+      acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+    } else if (t < NV) {
+      const int p = t - 1;
       const double* wK = rec + 2 + q;
       const double* sv = wK + P;
-      const double* Tm = sv + P;
-      for (int p = 0; p < P; ++p) {
-        double wz = sv[p];
-        for (int c = 0; c < q; ++c) wz -= Tm[p * q + c] * bet[c];
-        accv[1 + p] += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
-      }
+      const double* Tm = sv + P + p * q;
+      double wz = sv[p];
+      for (int c = 0; c < q; ++c) wz -= Tm[c] * beta[c];
+      acc += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
     }
   }
-  for (int e = 0; e < NV; ++e) {
-    red[threadIdx.x] = accv[e];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial2[(int64_t)blockIdx.x * NV + e] = red[0];
+  // sum over the workgroup's records, per lane-in-group t
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s >= G; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
+  if (threadIdx.x < NV) partial2[(int64_t)blockIdx.x * NV + threadIdx.x] = red[threadIdx.x];
 }
 
 __global__ __launch_bounds__(256) void ll_reduce2(const double* __restrict__ partial2, int G2,
@@ -2030,7 +2031,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   int Gmax = 1;
   for (const Cand& c : cands) Gmax = std::max(Gmax, c.G);
   int G = cands[dflt].G;
-  const int G2 = (int)std::min<int64_t>((vs.nblk + 255) / 256, 512);
+  // pass 2: lanes per record and workgroups
+  int G2L = 1;
+  if (need_grad)
+    while (G2L < 1 + P) G2L <<= 1;
+  SVH_REQUIRE(G2L <= 256, "pass 2: too many parameters");
+  const int G2 = (int)std::min<int64_t>((vs.nblk * G2L + 255) / 256, 2048);
   const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC) * sizeof(double) +
                      (size_t)KP * sizeof(int);
   SVH_REQUIRE(lds <= 160 * 1024, "eval: block too large for LDS");
@@ -2139,7 +2145,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     if (NACC > 0)
       hipLaunchKernelGGL(ll_sum_partials, dim3(NACC), dim3(256), 0, s, a.partial, G, NACC, tot);
     hipLaunchKernelGGL(ll_solve_beta, dim3(1), dim3(256), 0, s, tot, P, q, need_grad, dbeta, dout);
-    hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), 0, s, a.cache, vs.nblk, P, q, need_grad,
+    hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), 0, s, a.cache, vs.nblk, P, q, need_grad, G2L,
                        dbeta, partial2);
     hipLaunchKernelGGL(ll_reduce2, dim3(NV + 1), dim3(256), 0, s, partial2, G2, NV, fail, nchunk,
                        P, q, dout);
