@@ -1504,6 +1504,11 @@ __global__ __launch_bounds__(256) void ll_beta_batch(const DSDesc* __restrict__ 
                      D.P, kBP);
 }
 
+// Pass 2 of the batch: kB2L lanes per record as in ll_pass2 (lane t: value
+// term for t = 0, derivative t - 1 below 1 + P).
+constexpr int kB2L = 16;
+static_assert(kB2L >= 1 + kBP, "pass-2 lanes per record");
+
 __global__ __launch_bounds__(256) void ll_pass2_batch(const DSDesc* __restrict__ ds,
                                                       const FSState* __restrict__ fs,
                                                       const double* __restrict__ cache,
@@ -1514,35 +1519,36 @@ __global__ __launch_bounds__(256) void ll_pass2_batch(const DSDesc* __restrict__
   if (!fs_evaluates(fs[i].status)) return;
   const DSDesc D = ds[i];
   const int P = D.P, q = D.q;
-  double bet[kBQ];
-  for (int c = 0; c < kBQ; ++c) bet[c] = c < q ? beta[i * kBQ + c] : 0.0;
-  double accv[kBNV];
-  for (int e = 0; e < kBNV; ++e) accv[e] = 0.0;
-  for (int64_t b = D.blk0 + blockIdx.x * 256 + threadIdx.x; b < D.blk0 + D.nblk;
-       b += (int64_t)G2X * 256) {
+  const double* bet = beta + i * kBQ;
+  const int t = threadIdx.x & (kB2L - 1);
+  const int64_t rstride = (int64_t)G2X * (256 / kB2L);
+  double acc = 0.0;
+  for (int64_t b = D.blk0 + ((int64_t)blockIdx.x * 256 + threadIdx.x) / kB2L;
+       b < D.blk0 + D.nblk; b += rstride) {
     const double* rec = cache + b * kBNC;
     double zK = rec[0];
     for (int c = 0; c < q; ++c) zK -= rec[1 + c] * bet[c];
-    accv[0] += -0.5 * (2.0 * rec[1 + q] + zK * zK);
-    const double* wK = rec + 2 + q;
-    const double* sv = wK + P;
-    const double* Tm = sv + P;
-    for (int p = 0; p < P; ++p) {
+    if (t == 0) {
+      acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+    } else if (t <= P) {
+      const int p = t - 1;
+      const double* wK = rec + 2 + q;
+      const double* sv = wK + P;
+      const double* Tm = sv + P + p * q;
       double wz = sv[p];
-      for (int c = 0; c < q; ++c) wz -= Tm[p * q + c] * bet[c];
-      accv[1 + p] += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
+      for (int c = 0; c < q; ++c) wz -= Tm[c] * bet[c];
+      acc += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
     }
   }
-  for (int e = 0; e < kBNV; ++e) {
-    red[threadIdx.x] = accv[e];
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial2[((int64_t)i * G2X + blockIdx.x) * kBNV + e] = red[0];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s >= kB2L; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
+  if (threadIdx.x < kBNV)
+    partial2[((int64_t)i * G2X + blockIdx.x) * kBNV + threadIdx.x] =
+        threadIdx.x <= P ? red[threadIdx.x] : 0.0;
 }
 
 __global__ __launch_bounds__(256) void ll_reduce2_batch(const DSDesc* __restrict__ ds,
@@ -2274,7 +2280,8 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   // workgroups per CU in pass 1, as the single-dataset launch uses)
   const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
   B.GX = (int)std::min<int64_t>(maxn, std::max(1024, (8 * ncu + ndata - 1) / ndata));
-  B.G2X = (int)std::min<int64_t>((maxn + 255) / 256, std::max(64, (2 * ncu + ndata - 1) / ndata));
+  B.G2X = (int)std::min<int64_t>((maxn * kB2L + 255) / 256,
+                                  std::max(64, (8 * ncu + ndata - 1) / ndata));
   // block tables (rows are 0..n-1 within each dataset, in order)
   std::vector<int> hrows(bo), hnbr((size_t)bo * mmax, -1), hk(bo), hc(bo), chunk_ds(co);
   for (int i = 0; i < ndata; ++i) {
