@@ -792,10 +792,32 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A double moved between lanes by a DPP control (both dwords), all lanes active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each half-wave (32 lanes), every lane gets its half's sum; all
+// lanes active.  VALU lane moves only: quad permutes (lane ^ 1, lane ^ 2),
+// rotations by 4 and 8 inside each 16-lane row, then the row pair swap of
+// v_permlane16_swap (rows 0/1 and 2/3), whose two outputs summed give
+// row 0 + row 1 (row 2 + row 3) in every lane of the pair.
 __device__ __forceinline__ double half_sum(double v) {
-#pragma unroll
-  for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 32);
-  return v;
+  v += dpp_double<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_double<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_double<0x124>(v);   // row_ror:4
+  v += dpp_double<0x128>(v);   // row_ror:8
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const double a0 = __longlong_as_double(((long long)sh[0] << 32) | sl[0]);
+  const double a1 = __longlong_as_double(((long long)sh[1] << 32) | sl[1]);
+  return a0 + a1;
 }
 
 // LDS doubles per wave of the rows kernel: per half [cb 32 | bb 16 | ab 32 |
