@@ -820,11 +820,47 @@ __device__ __forceinline__ double half_sum(double v) {
   return a0 + a1;
 }
 
+// v of lane SRC of this lane's half (SRC in 0..31), for both halves at once;
+// all lanes active.  row_newbcast gives every 16-lane row its lane SRC % 16,
+// the row pair swap then hands row SRC / 16 of each half to both of its rows.
+template <int SRC>
+__device__ __forceinline__ double half_bcast(double v) {
+  static_assert(SRC >= 0 && SRC < 32, "source lane of a half");
+  const double t = __builtin_amdgcn_update_dpp(0.0, v, 0x150 + (SRC & 15), 0xF, 0xF, false);
+  const long long b = __double_as_longlong(t);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  constexpr int o = SRC < 16 ? 0 : 1;
+  return __longlong_as_double(((long long)sh[o] << 32) | sl[o]);
+}
+
 // LDS doubles per wave of the rows kernel: per half [cb 32 | bb 16 | ab 32 |
-// wb 16 | xs 32*dm] behind one base, then misc (16) and Mt (24 x 33)
+// (unused) 16 | xs 32*dm] behind one base, then misc (16) and Mt (24 x 33)
 __host__ __device__ inline int rows_half_doubles(int dm) { return 96 + 32 * dm; }
 __host__ __device__ inline int rows_lds_doubles(int dm) {
   return 2 * rows_half_doubles(dm) + 16 + 24 * 33;
+}
+
+// Step K of W = L'^-1 (dS a) and the steps after it: row K's right-hand
+// sides (lane K of each half) broadcast by lane moves, eliminated below the
+// diagonal.
+template <int K>
+__device__ __forceinline__ void w_sweep(double (&R)[kRC], const double (&S)[32], int r) {
+  const double l = (r > K) ? S[K] : 0.0;
+  // four columns at a time (bounded register pressure)
+#pragma unroll
+  for (int c0 = 0; c0 < kRC; c0 += 4) {
+    double wk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wk[u] = half_bcast<K>(R[c0 + u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) R[c0 + u] = fma(-l, wk[u], R[c0 + u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(R[c0 + u]));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (K + 1 < 31) w_sweep<K + 1>(R, S, r);
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
@@ -840,7 +876,6 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   double* cb = hb;                                    // column broadcast
   double* bb = hb + 32;                               // right-hand-side row broadcast
   double* ab = hb + 48;                               // a
-  double* wb = hb + 80;                               // derivative row broadcast
   double* xs = hb + 96;                               // coordinates of this half's block
   double* misc = lds + 2 * rows_half_doubles(DM);     // per half: [8h] fail, [8h+1..2] pivot
   double* Mt = misc + 16;                             // 24 x 33 product operands
@@ -978,23 +1013,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     }
     SVH_PHASE(4)
     // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
-#pragma unroll
-    for (int k = 0; k < 31; ++k) {
-      if (r == k) {
-#pragma unroll
-        for (int c = 0; c < kRC; ++c) wb[c] = R[c];
-      }
-      wave_sync();
-      if (r > k) {
-        const double l = S[k];
-#pragma unroll
-        for (int c = 0; c < kRC; ++c) R[c] = fma(-l, wb[c], R[c]);
-      }
-#pragma unroll
-      for (int c = 0; c < kRC; ++c) asm volatile("" : "+v"(R[c]));
-      wave_sync();
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    w_sweep<0>(R, S, r);
 #pragma unroll
     for (int c = 0; c < kRC; ++c) R[c] *= dinv;
     SVH_PHASE(5)
