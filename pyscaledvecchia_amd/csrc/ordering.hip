@@ -1019,6 +1019,8 @@ void launch_knn_t(svh_ctx* ctx, const std::vector<KNNDesc>& h, int64_t maxq, int
     hipLaunchKernelGGL((knn_tiled_kernel<4, LCAP>), gt, bt, 0, ctx->stream, dd);
   else if (dmax <= 8)
     hipLaunchKernelGGL((knn_tiled_kernel<8, LCAP>), gt, bt, 0, ctx->stream, dd);
+  else if (dmax <= 10)
+    hipLaunchKernelGGL((knn_tiled_kernel<10, LCAP>), gt, bt, 0, ctx->stream, dd);
   else if (dmax <= kMMD)
     hipLaunchKernelGGL((knn_tiled_kernel<kMMD, LCAP>), gt, bt, 0, ctx->stream, dd);
   else
@@ -1105,8 +1107,8 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   SVH_HIP(hipGetLastError());
   if (maxn <= 1) return;
   SVH_REQUIRE(dmax <= kMMD, "maximin: d above 12 not supported");
-  // coordinate registers per point: d rounded up to 4, 8 or 12 (zero padded)
-  const int dm = dmax <= 4 ? 4 : (dmax <= 8 ? 8 : 12);
+  // coordinate registers per point: d rounded up to 4, 8, 10 or 12 (zero padded)
+  const int dm = dmax <= 4 ? 4 : (dmax <= 8 ? 8 : (dmax <= 10 ? 10 : 12));
   int sortcap = 1;
   while (sortcap < maxslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
@@ -1137,6 +1139,9 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
       else if (dm == 8)
         hipLaunchKernelGGL(mm_apply_kernel<8>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
                            acc, accx, st, ck, ci);
+      else if (dm == 10)
+        hipLaunchKernelGGL(mm_apply_kernel<10>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
+                           acc, accx, st, ck, ci);
       else
         hipLaunchKernelGGL(mm_apply_kernel<12>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
                            acc, accx, st, ck, ci);
@@ -1148,6 +1153,9 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
       else if (dm == 8)
         hipLaunchKernelGGL(mm_select_kernel<8>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
                            dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
+      else if (dm == 10)
+        hipLaunchKernelGGL(mm_select_kernel<10>, dim3(nd), dim3(kPool), select_lds, ctx->stream,
+                           dd, dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
       else
         hipLaunchKernelGGL(mm_select_kernel<12>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
                            dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);

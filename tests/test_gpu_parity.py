@@ -22,7 +22,8 @@ def _rel(a, b):
 
 # ------------------------------------------------------------------ (a)
 @pytest.mark.parametrize("n,d,seed", [(1, 3, 0), (2, 2, 1), (57, 3, 2), (300, 4, 3),
-                                      (2000, 8, 4), (5000, 3, 5)])
+                                      (2000, 8, 4), (5000, 3, 5), (3000, 10, 6), (1500, 9, 7),
+                                      (1200, 12, 8)])
 def test_maximin_bit_exact(gpu, n, d, seed):
     X = np.random.default_rng(seed).random((n, d)) / 0.3
     first = O.maximin_first(X)
@@ -57,7 +58,8 @@ def test_maximin_large_prefix_and_property(gpu):
 
 @pytest.mark.parametrize("n,d,m,start,seed", [(120, 3, 8, 0, 2), (2000, 4, 30, 0, 7),
                                               (600, 8, 30, 400, 8), (60, 2, 6, 40, 4),
-                                              (300, 1, 5, 0, 9), (200, 5, 140, 0, 10)])
+                                              (300, 1, 5, 0, 9), (200, 5, 140, 0, 10),
+                                              (800, 10, 30, 0, 11), (500, 9, 140, 100, 12)])
 def test_ordered_knn_bit_exact(gpu, n, d, m, start, seed):
     X = np.random.default_rng(seed).random((n, d))
     np.testing.assert_array_equal(gpu.ordered_knn(X, m, start), O.find_ordered_nn(X, m, start))
