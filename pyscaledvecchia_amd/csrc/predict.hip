@@ -39,6 +39,21 @@
 
 namespace {
 
+// v of lane `src` of this lane's 16-lane row (DPP row_newbcast); src is a
+// compile-time constant after unrolling, the switch folds away.
+__device__ __forceinline__ double row_bcast(double v, int src) {
+  switch (src) {
+#define SVH_RB(n) \
+  case n:         \
+    return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + n, 0xF, 0xF, false);
+    SVH_RB(0) SVH_RB(1) SVH_RB(2) SVH_RB(3) SVH_RB(4) SVH_RB(5) SVH_RB(6) SVH_RB(7)
+    SVH_RB(8) SVH_RB(9) SVH_RB(10) SVH_RB(11) SVH_RB(12) SVH_RB(13) SVH_RB(14) SVH_RB(15)
+#undef SVH_RB
+    default:
+      return 0.0;
+  }
+}
+
 constexpr int kWave = 64;
 
 __constant__ double kJitDevP[10] = {0.0,  1e-12, 1e-11, 1e-10, 1e-09,
@@ -325,10 +340,10 @@ __global__ __launch_bounds__(kPT) void pred_mfma_kernel(PredArgs a, PredTileArgs
             P[(16 * (tI[s] - J) + acc_row<MIX>(lk, v)) * kPStride + li] = T[s][v];
         }
       __syncthreads();
-      // diagonal tile: FP64, lane r holds row r & 15 (four copies per wave).
-      // Step c: the rows' current column-c values go through LDS (one
-      // write, broadcast reads of all 16; a wave's LDS operations complete in
-      // order), the pivot among them; buffers alternate between steps.
+      // diagonal tile: FP64, lane r holds row r & 15 (four copies per wave,
+      // one per 16-lane row).  Step c: the rows' current column-c values are
+      // broadcast inside each 16-lane row by DPP row_newbcast (no LDS), the
+      // pivot among them.
       if (wid == 0) {
         const int r = li;
         double R[16];
@@ -338,11 +353,9 @@ __global__ __launch_bounds__(kPT) void pred_mfma_kernel(PredArgs a, PredTileArgs
         bool f = false;
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
-          double* cb = misc + 32 + 16 * (c & 1);
-          if (lane < 16) cb[r] = R[c];
           double g[16];
 #pragma unroll
-          for (int c2 = c; c2 < 16; ++c2) g[c2] = cb[c2];
+          for (int c2 = c; c2 < 16; ++c2) g[c2] = row_bcast(R[c], c2);
           const double piv = g[c];
           const double lpp = sqrt(piv);
           const double inv = 1.0 / lpp;
