@@ -357,8 +357,14 @@ __global__ __launch_bounds__(kPT) void pred_mfma_kernel(PredArgs a, PredTileArgs
 #pragma unroll
           for (int c2 = c; c2 < 16; ++c2) g[c2] = row_bcast(R[c], c2);
           const double piv = g[c];
-          const double lpp = sqrt(piv);
-          const double inv = 1.0 / lpp;
+          // 1 / sqrt(piv) by the hardware estimate and three Newton steps
+          // (faithful to an ulp), sqrt(piv) = piv / sqrt(piv)
+          double inv = __builtin_amdgcn_rsq(piv);
+          const double h = 0.5 * piv;
+          inv = inv * fma(-h * inv, inv, 1.5);
+          inv = inv * fma(-h * inv, inv, 1.5);
+          inv = inv * fma(-h * inv, inv, 1.5);
+          const double lpp = piv * inv;
           if (16 * J + c < Kb && !(piv > 0.0)) f = true;
           if (16 * J + c == Kb - 1) dg = lpp;   // L_KK
           R[c] = (r > c) ? R[c] * inv : ((r == c) ? lpp : R[c]);
