@@ -1656,8 +1656,8 @@ __device__ bool lu_solve(double* A, double* b, int n) {
 // Fisher-scoring direction, ridged until it is an ascent direction
 // (optimize.py:28-45), capped at norm 3 (optimize.py:47-49); sets the trial.
 __device__ void fs_begin_search(FSState& F, int P) {
-  double A[kBP * kBP], x[kBP];
-  double dg[kBP];
+  // one thread runs this: its work arrays live in LDS
+  __shared__ double A[kBP * kBP], x[kBP], dg[kBP];
   for (int p = 0; p < P; ++p) dg[p] = fmax(F.M[p * P + p], 1e-8);
   double ridge = 0.0;
   bool have = false;
@@ -1714,14 +1714,11 @@ __device__ void fs_begin_iteration(FSState& F, int P) {
 }
 
 // One workgroup per dataset: escalate failed chunks, then advance the state.
-__global__ __launch_bounds__(64) void fs_update(const DSDesc* __restrict__ ds,
-                                                FSState* __restrict__ fs,
-                                                const double* __restrict__ out,
-                                                const double* __restrict__ beta,
-                                                int* __restrict__ level, int* __restrict__ fail,
-                                                double var_penalty, int* __restrict__ status_out) {
-  const int i = blockIdx.x;
-  FSState& F = fs[i];
+__device__ __forceinline__ void fs_update_body(FSState& F, const DSDesc* __restrict__ ds, int i,
+                                               const double* __restrict__ out,
+                                               const double* __restrict__ beta,
+                                               int* __restrict__ level, int* __restrict__ fail,
+                                               double var_penalty, int* __restrict__ status_out) {
   const DSDesc D = ds[i];
   const int P = D.P;
   __shared__ int st0;
@@ -1785,7 +1782,7 @@ __global__ __launch_bounds__(64) void fs_update(const DSDesc* __restrict__ ds,
   double thc0 = th[0];
   const double LOG2PI = 1.8378770664093453;
   double ll = -0.5 * (double)D.nblk * LOG2PI + o[0];
-  double g[kBP], M[kBP * kBP];
+  __shared__ double g[kBP], M[kBP * kBP];
   for (int p = 0; p < P; ++p) g[p] = o[1 + p];
   for (int e = 0; e < P * P; ++e) M[e] = o[1 + P + e];
   if (var_penalty > 0.0) {
@@ -1840,6 +1837,28 @@ __global__ __launch_bounds__(64) void fs_update(const DSDesc* __restrict__ ds,
     }
   }
   status_out[i] = F.status;
+}
+
+// One workgroup per dataset: the dataset's state is copied into LDS by the
+// whole workgroup, advanced there (thread 0 for the serial parts), and copied
+// back.
+__global__ __launch_bounds__(64) void fs_update(const DSDesc* __restrict__ ds,
+                                                FSState* __restrict__ fs,
+                                                const double* __restrict__ out,
+                                                const double* __restrict__ beta,
+                                                int* __restrict__ level, int* __restrict__ fail,
+                                                double var_penalty, int* __restrict__ status_out) {
+  static_assert(sizeof(FSState) % sizeof(unsigned long long) == 0, "state copy in words");
+  constexpr int kW = sizeof(FSState) / sizeof(unsigned long long);
+  __shared__ FSState Fs;
+  const int i = blockIdx.x;
+  unsigned long long* gw = reinterpret_cast<unsigned long long*>(fs + i);
+  unsigned long long* lw = reinterpret_cast<unsigned long long*>(&Fs);
+  for (int w = threadIdx.x; w < kW; w += 64) lw[w] = gw[w];
+  __syncthreads();
+  fs_update_body(Fs, ds, i, out, beta, level, fail, var_penalty, status_out);
+  __syncthreads();
+  for (int w = threadIdx.x; w < kW; w += 64) gw[w] = lw[w];
 }
 
 template <int KP>
