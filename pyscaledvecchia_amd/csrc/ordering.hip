@@ -885,17 +885,23 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <int DM, int LCAP>
-__global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
+__global__ __launch_bounds__(kWave * kKnnWaves, 4) void knn_tiled_kernel(
     const KNNDesc* __restrict__ descs) {
   constexpr int E = LCAP / kWave;
   const KNNDesc D = descs[blockIdx.y];
-  const double* __restrict__ XT = D.XT;
+  // global address space: the tile loads count on vmcnt only (flat loads
+  // would also hold up every LDS wait of the scan)
+  typedef const double __attribute__((address_space(1)))* gptr;
+  const gptr XT = (gptr)D.XT;
   const int64_t nref = D.nref, rowstart = D.rowstart;
   const int d = D.d, m = D.m, ordered = D.ordered, out_stride = D.out_stride;
   const int64_t nqry = ordered ? nref : D.nq;
   const int64_t q0 = (int64_t)blockIdx.x * kKnnWaves;
   if (q0 >= nqry) return;
-  __shared__ double tile[DM * kKnnTile];
+  constexpr int NTH = kWave * kKnnWaves;
+  constexpr int PER = DM * kKnnTile / NTH;   // tile values loaded per thread
+  static_assert(PER * NTH == DM * kKnnTile, "tile split");
+  __shared__ double tile[2][DM * kKnnTile];
   __shared__ double lkey[kKnnWaves][LCAP];
   __shared__ int lidx[kKnnWaves][LCAP];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -923,7 +929,7 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
   if (ordered && qlast < rowstart) return;
   const int64_t hwg = ordered ? qlast : nref;
   double qx[DM];
-  const double* qcol = ordered ? XT : D.QT;
+  const double* qcol = ordered ? D.XT : D.QT;
 #pragma unroll
   for (int l = 0; l < DM; ++l) qx[l] = (qvalid && l < d) ? qcol[(int64_t)l * nqry + q] : 0.0;
   const double kInf = __longlong_as_double(0x7ff0000000000000ll);
@@ -961,23 +967,39 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
     nbuf = 0;
   };
 
-  for (int64_t base = 0; base < hwg; base += kKnnTile) {
-    __syncthreads();   // the previous tile is consumed
-    for (int e = threadIdx.x; e < DM * kKnnTile; e += kWave * kKnnWaves) {
-      const int l = e / kKnnTile, t = e - l * kKnnTile;
-      const int64_t j = base + t;
-      tile[e] = (l < d && j < nref) ? XT[(int64_t)l * nref + j] : 0.0;
+  // tiles are double-buffered: thread t holds elements t + r * NTH of the
+  // next tile in registers while the current one is scanned
+  double pre[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int e = threadIdx.x + r * NTH;
+    const int l = e / kKnnTile, t = e - l * kKnnTile;
+    pre[r] = (hwg > 0 && l < d && t < nref) ? XT[(int64_t)l * nref + t] : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < PER; ++r) tile[0][threadIdx.x + r * NTH] = pre[r];
+  __syncthreads();
+  int buf = 0;
+  for (int64_t base = 0; base < hwg; base += kKnnTile, buf ^= 1) {
+    const bool more = base + kKnnTile < hwg;
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        const int e = threadIdx.x + r * NTH;
+        const int l = e / kKnnTile, t = e - l * kKnnTile;
+        const int64_t j = base + kKnnTile + t;
+        pre[r] = (l < d && j < nref) ? XT[(int64_t)l * nref + j] : 0.0;
+      }
     }
-    __syncthreads();
-    if (k <= 0) continue;
+    const double* tb = tile[buf];
     const int64_t rem = hi - base;
-    const int tend = rem < kKnnTile ? (int)rem : kKnnTile;
+    const int tend = k <= 0 ? 0 : (rem < kKnnTile ? (int)rem : kKnnTile);
     for (int c = 0; c < tend; c += kWave) {
       const int t = c + lane;
       const int j = (int)(base + t);
       double s = 0.0;
 #pragma unroll
-      for (int l = 0; l < DM; ++l) s = sq_step(s, qx[l], tile[l * kKnnTile + t]);
+      for (int l = 0; l < DM; ++l) s = sq_step(s, qx[l], tb[l * kKnnTile + t]);
       bool pass = t < tend && nn_less(s, j, thr, thr_i);
       unsigned long long msk = __ballot(pass);
       int cnt = __popcll(msk);
@@ -995,6 +1017,14 @@ __global__ __launch_bounds__(kWave * kKnnWaves) void knn_tiled_kernel(
       }
       nbuf += cnt;
     }
+    // the scan above is done before the other buffer is written (it was last
+    // read before the previous barrier)
+    __builtin_amdgcn_sched_barrier(0);
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < PER; ++r) tile[buf ^ 1][threadIdx.x + r * NTH] = pre[r];
+    }
+    __syncthreads();
   }
   if (k <= 0) return;
   merge();
