@@ -896,7 +896,10 @@ __global__ __launch_bounds__(kWave * kKnnWaves, 4) void knn_tiled_kernel(
   const int64_t nref = D.nref, rowstart = D.rowstart;
   const int d = D.d, m = D.m, ordered = D.ordered, out_stride = D.out_stride;
   const int64_t nqry = ordered ? nref : D.nq;
-  const int64_t q0 = (int64_t)blockIdx.x * kKnnWaves;
+  // ordered form: the last rows scan the most candidates, so workgroups take
+  // row groups from the end (longest first, for the balance of the tail)
+  const int64_t wg = ordered ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
+  const int64_t q0 = wg * kKnnWaves;
   if (q0 >= nqry) return;
   constexpr int NTH = kWave * kKnnWaves;
   constexpr int PER = DM * kKnnTile / NTH;   // tile values loaded per thread
