@@ -178,7 +178,10 @@ int svh_joint_solve(svh_ctx* ctx, int64_t nobs, int64_t ns, int32_t K, const int
  *              neighbour indices, stride mmax; chunk ids local to the dataset
  *              as _covariance.py:255-259 slices them).  lvt = log variance
  *              target of the sigma^2 penalty; fixed_nug/log_nug hold a fixed
- *              nugget (gp.py:196-208).
+ *              nugget (gp.py:196-208).  X, y, Z, nbr and kcnt may be NULL
+ *              when every dataset is then built on the device
+ *              (svh_batch_data + svh_batch_rebuild): the buffers are
+ *              allocated, neighbour lists empty.
  *   update:    replace dataset i's ordered inputs/neighbours after a rebuild.
  *   fit_start: initial log-parameters, state machines reset; use_graph != 0
  *              captures one step (evaluation + update) in a hipGraph.

@@ -468,17 +468,22 @@ class BatchHandle:
 
     def setup(self, d, q, nu_code, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt, fixed_nug,
               log_nug, lambda_max):
+        """X, y, Z and nbr, kcnt may be None: buffers only, filled by
+        data() + rebuild()."""
         self.n = _i64(n)
         self.ndata = len(self.n)
-        X, y = _finite(X, "X"), _finite(y, "y")
+        if X is not None:
+            X, y = _finite(X, "X"), _finite(y, "y")
         Z = _f64(Z) if Z is not None and Z.size else np.zeros(1)
+        nbr = None if nbr is None else _i64(nbr)
+        kcnt = None if kcnt is None else _i32(kcnt)
         args = [_i32(d), _i32(q), _i32(nu_code), self.n]
         self._keep = args
         c = self.ctx
         c._check(self._L.svh_batch_setup(
             c._h, self.ndata, _ptr(args[0], _i32p), _ptr(args[1], _i32p), _ptr(args[2], _i32p),
-            _ptr(args[3], _i64p), _ptr(X), _ptr(y), _ptr(Z), _ptr(_i64(nbr), _i64p),
-            _ptr(_i32(kcnt), _i32p), _ptr(_i32(chunk), _i32p), _ptr(_i32(nchunk), _i32p),
+            _ptr(args[3], _i64p), _ptr(X), _ptr(y), _ptr(Z), _ptr(nbr, _i64p),
+            _ptr(kcnt, _i32p), _ptr(_i32(chunk), _i32p), _ptr(_i32(nchunk), _i32p),
             int(mmax), _ptr(_f64(lvt)), _ptr(_i32(fixed_nug), _i32p), _ptr(_f64(log_nug)),
             float(lambda_max)))
 

@@ -85,8 +85,8 @@ class BatchedFit:
                         th[-1] = math.log(models[i].nugget)
                     preps[i]["theta"] = models[i]._clip_theta(th, preps[i]["d"])
         mmax = max(p["m"] for p in preps)
-        # the batch is set up with the unordered data and empty neighbour
-        # lists; every structure is then built on the device
+        # the batch buffers are allocated empty (no ordered data, no
+        # neighbour lists); every structure is then built on the device
         d = np.array([p["d"] for p in preps], np.int32)
         q = np.array([p["Ze"].shape[1] for p in preps], np.int32)
         nu = np.array([_NU_CODE[g.nu] for g in models], np.int32)
@@ -95,8 +95,6 @@ class BatchedFit:
         y = np.concatenate([p["ye"] for p in preps])
         Z = (np.concatenate([np.ascontiguousarray(p["Ze"]).ravel() for p in preps]) if q.sum()
              else np.zeros(0))
-        nbr = np.full((int(n.sum()), mmax), -1, np.int64)
-        kcnt = np.zeros(int(n.sum()), np.int32)
         chunks = [_chunk_ids_rows(len(p["ye"]), p["m"], p["d"] + 2) for p in preps]
         chunk = np.concatenate([c for c, _ in chunks])
         nchunk = np.array([nc for _, nc in chunks], np.int32)
@@ -104,8 +102,8 @@ class BatchedFit:
         fixed = np.array([g.nugget is not None for g in models], np.int32)
         lnug = np.array([math.log(g.nugget) if g.nugget is not None else 0.0 for g in models])
         bh = _lib.BatchHandle(ctx)
-        bh.setup(d, q, nu, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt, fixed, lnug,
-                 models[0].lambda_max)
+        bh.setup(d, q, nu, n, None, None, None, None, None, chunk, nchunk, mmax, lvt, fixed,
+                 lnug, models[0].lambda_max)
         bh.data(X, y, Z)
         self._rebuild(bh, preps, list(range(nd)), [p["theta"] for p in preps])
         t_build = time.perf_counter() - t0

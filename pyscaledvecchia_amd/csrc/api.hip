@@ -315,9 +315,11 @@ int svh_batch_setup(svh_ctx* ctx, int32_t ndata, const int32_t* d, const int32_t
                     const int32_t* chunk, const int32_t* nchunk, int32_t mmax, const double* lvt,
                     const int32_t* fixed_nug, const double* log_nug, double lambda_max) {
   return guarded(ctx, [&] {
-    SVH_REQUIRE(d && q && nu_code && n && X && y && nbr && kcnt && chunk && nchunk && lvt &&
-                    fixed_nug && log_nug,
+    SVH_REQUIRE(d && q && nu_code && n && chunk && nchunk && lvt && fixed_nug && log_nug,
                 "batch_setup: null pointer");
+    SVH_REQUIRE((X == nullptr) == (y == nullptr), "batch_setup: X and y both given or both NULL");
+    SVH_REQUIRE((nbr == nullptr) == (kcnt == nullptr),
+                "batch_setup: nbr and kcnt both given or both NULL");
     batch_setup_device(ctx, ndata, d, q, nu_code, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt,
                        fixed_nug, log_nug, lambda_max);
   });

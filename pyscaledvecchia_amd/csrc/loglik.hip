@@ -2343,16 +2343,20 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   B.G2X = (int)std::min<int64_t>((maxn * kB2L + 255) / 256,
                                   std::max(64, (8 * ncu + ndata - 1) / ndata));
   // block tables (rows are 0..n-1 within each dataset, in order)
-  std::vector<int> hrows(bo), hnbr((size_t)bo * mmax, -1), hk(bo), hc(bo), chunk_ds(co);
+  // (nbr / kcnt NULL: empty lists, filled later by device rebuilds)
+  const bool lists = nbr != nullptr;
+  std::vector<int> hrows(bo), hnbr(lists ? (size_t)bo * mmax : 0, -1), hk(bo, 0), hc(bo),
+      chunk_ds(co);
   for (int i = 0; i < ndata; ++i) {
     const DSDesc& D = B.desc[i];
     for (int64_t r = 0; r < D.nblk; ++r) {
       const int64_t b = D.blk0 + r;
       hrows[b] = (int)r;
-      hk[b] = kcnt[b];
-      SVH_REQUIRE(kcnt[b] >= 0 && kcnt[b] <= mmax, "batch: bad neighbour count");
       SVH_REQUIRE(chunk[b] >= 0 && chunk[b] < D.nchunk, "batch: bad chunk id");
       hc[b] = (int)(D.chunk0 + chunk[b]);
+      if (!lists) continue;
+      hk[b] = kcnt[b];
+      SVH_REQUIRE(kcnt[b] >= 0 && kcnt[b] <= mmax, "batch: bad neighbour count");
       for (int t = 0; t < kcnt[b]; ++t) {
         const int64_t v = nbr[b * mmax + t];
         SVH_REQUIRE(v >= 0 && v < D.nblk, "batch: neighbour index out of range");
@@ -2366,12 +2370,23 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
     void* dst = ctx->arena.get(name, bytes);
     if (bytes) SVH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
   };
-  up("b.X", X, (size_t)xo * sizeof(double));
-  up("b.y", y, (size_t)yo * sizeof(double));
-  if (zo > 0) up("b.Z", Z, (size_t)zo * sizeof(double));
-  else ctx->arena.get("b.Z", 8);
+  if (X) {
+    up("b.X", X, (size_t)xo * sizeof(double));
+    up("b.y", y, (size_t)yo * sizeof(double));
+    if (zo > 0) up("b.Z", Z, (size_t)zo * sizeof(double));
+  } else {
+    ctx->arena.get("b.X", (size_t)xo * sizeof(double));
+    ctx->arena.get("b.y", (size_t)yo * sizeof(double));
+    if (zo > 0) ctx->arena.get("b.Z", (size_t)zo * sizeof(double));
+  }
+  if (zo == 0) ctx->arena.get("b.Z", 8);
   up("b.rows", hrows.data(), hrows.size() * sizeof(int));
-  up("b.nbr", hnbr.data(), hnbr.size() * sizeof(int));
+  if (lists) {
+    up("b.nbr", hnbr.data(), hnbr.size() * sizeof(int));
+  } else {
+    int* dnbr = (int*)ctx->arena.get("b.nbr", (size_t)bo * mmax * sizeof(int));
+    int_fill(ctx, dnbr, -1, bo * mmax);
+  }
   up("b.kcnt", hk.data(), hk.size() * sizeof(int));
   up("b.chunk", hc.data(), hc.size() * sizeof(int));
   up("b.chunk_ds", chunk_ds.data(), chunk_ds.size() * sizeof(int));
