@@ -30,9 +30,20 @@ PINS = {
     "vecchia_profile_loglik": "m=n-1 equals the dense exact GP loglik; FD gradient; "
                               "Fisher symmetric/PSD and equal to the dense "
                               "1/2 tr(K^-1 K_j K^-1 K_k) at m=n-1; beta equals dense GLS",
-    "fisher_scoring": "parity unpinned (trajectory); converges to a stationary point "
-                      "(gradient small) on toy problems",
+    "fisher_scoring": "the reference's own trajectories: every objective argument, "
+                      "reorder-hook call and result of optimize.py:11-73 on eleven "
+                      "synthetic objectives that reach each branch (ridge ladder, "
+                      "LinAlgError in the solve and in the line search, gradient "
+                      "fallback, non-finite step and loglik, step cap, failed search, "
+                      "reorder, max_iter), tests/golden/fisher_scoring.npz",
     "predict_scaled": "dense conditional Gaussian on the m nearest neighbours",
-    "predict_joint": "exact dense GP joint conditional when m >= n + ns - 1",
-    "estimate_b": "closed form mean((y-mu)^2/v)",
+    "predict_joint": "exact dense GP conditional for one point with m >= n; for "
+                     "several points the dense Vecchia joint law written from its "
+                     "definition (conditional regressions on brute-force neighbour "
+                     "sets, noise-free prediction neighbours): mean, variance and "
+                     "paths from the same normal draws",
+    "estimate_b": "numerical minimiser of the held-out score on a split rebuilt by "
+                  "hand from the RNG call order, dense marginal predictions; floor "
+                  "and n < 200 cases",
+    "mlp adam_step": "torch.optim.AdamW over six steps with weight decay",
 }

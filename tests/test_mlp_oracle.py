@@ -55,3 +55,24 @@ def test_adam_first_step_moves_by_lr():
     N.adam_step(p, g, m, v, 1, lr=0.01)
     # first bias-corrected step is lr * sign(g) (up to eps)
     np.testing.assert_allclose(p, [0.99, -1.99, 0.5], atol=1e-6)
+
+
+def test_adamw_steps_match_torch():
+    """Several bias-corrected AdamW steps with weight decay (t = 1..6) against
+    torch.optim.AdamW on the CPU (decoupled decay: p <- p - lr (m^/(sqrt v^ + eps)
+    + wd p))."""
+    import torch
+    rng = np.random.default_rng(4)
+    p0 = rng.standard_normal(50).astype(np.float32)
+    grads = [rng.standard_normal(50).astype(np.float32) * (0.1 + t) for t in range(6)]
+    lr, b1, b2, eps, wd = 3e-3, 0.9, 0.999, 1e-8, 1e-2
+    p = p0.copy()
+    m, v = np.zeros(50, np.float32), np.zeros(50, np.float32)
+    tp = torch.nn.Parameter(torch.tensor(p0.copy()))
+    opt = torch.optim.AdamW([tp], lr=lr, betas=(b1, b2), eps=eps, weight_decay=wd)
+    for t, g in enumerate(grads, start=1):
+        N.adam_step(p, g, m, v, t, lr=lr, beta1=b1, beta2=b2, eps=eps, wd=wd)
+        tp.grad = torch.tensor(g.copy())
+        opt.step()
+        np.testing.assert_allclose(p, tp.detach().numpy(), rtol=2e-6, atol=2e-7)
+    assert np.abs(p - p0).max() > 3 * lr      # the steps moved the parameters

@@ -348,3 +348,130 @@ def test_fisher_scoring_reaches_stationary_point():
 
 
 import scipy.spatial  # noqa: E402  (used above)
+
+
+# ---------------------------------------------------------------- b-step
+def _dense_marginal(Xtr, r_tr, Xq, theta, m, nu=2.5):
+    """Noise-free conditional mean/variance of each query given its m nearest
+    training points (brute-force kNN in the scaled space, dense solve)."""
+    d = Xtr.shape[1]
+    lam, var, nug = np.exp(theta[1:1 + d]), math.exp(theta[0]), math.exp(theta[-1])
+    mu, va = np.empty(len(Xq)), np.empty(len(Xq))
+    for i, x in enumerate(Xq):
+        dist = np.linalg.norm((Xtr - x) / lam, axis=1)
+        nb = np.argsort(dist, kind="stable")[:m]
+        P = np.vstack([Xtr[nb], x])
+        C = var * O.matern_corr(scipy.spatial.distance.cdist(P / lam, P / lam), nu)
+        C[np.arange(m), np.arange(m)] += var * nug
+        w = np.linalg.solve(C[:m, :m], C[:m, m])
+        mu[i] = w @ r_tr[nb]
+        va[i] = C[m, m] - w @ C[:m, m]
+    return mu, va
+
+
+def test_estimate_b_split_and_minimiser():
+    """b of the oracle fit equals the minimiser, found numerically, of the
+    held-out negative log score sum_i log(b v_i) + (y_i - mu_i)^2 / (b v_i)
+    (gp.py:410-425) on a split rebuilt by hand from the RNG call order:
+    default_rng(random_state) -> choice(n, n_est) when n > n_est (gp.py:159-161)
+    -> permutation(n), first min(2000, n // 5) held out (gp.py:413-415)."""
+    import scipy.optimize
+    X, y = _sine(300, seed=5)
+    y = y + 0.05 * np.random.default_rng(6).standard_normal(300)
+    gp = O.OracleGP(m_est=10, m_pred=25, n_est=200, trend="linear", random_state=3).fit(X, y)
+    rng = np.random.default_rng(3)
+    rng.choice(300, 200, replace=False)
+    idx = rng.permutation(300)
+    te, tr = idx[:60], idx[60:]
+    Z = O.design(gp.X_, "linear")
+    r_tr = gp.y_[tr] - Z[tr] @ gp.beta_
+    mu, va = _dense_marginal(gp.X_[tr], r_tr, gp.X_[te], gp.theta_, 25)
+    mu = mu + Z[te] @ gp.beta_
+    e2 = (gp.y_[te] - mu) ** 2
+
+    def score(lb):
+        b = math.exp(lb)
+        return float(np.sum(np.log(b * va) + e2 / (b * va)))
+    res = scipy.optimize.minimize_scalar(score, bracket=(-3.0, 3.0), tol=1e-12)
+    assert gp.b_ == pytest.approx(math.exp(res.x), rel=1e-6)
+    # a different split (no subsample draw first) gives a different b
+    rng2 = np.random.default_rng(3)
+    te2 = rng2.permutation(300)[:60]
+    assert not np.array_equal(np.sort(te2), np.sort(te))
+
+
+def test_estimate_b_floor_and_small_n():
+    X, y = _sine(150, seed=7)
+    gp = O.OracleGP(m_est=8, m_pred=15, n_est=150).fit(X, y)
+    assert gp.b_ == 1.0                       # n < 200: no correction (gp.py:237)
+    X, y = _sine(250, seed=7)
+    gp = O.OracleGP(m_est=8, m_pred=15, n_est=250, trend="linear").fit(X, y)
+    gp.y_ = O.design(gp.X_, "linear") @ gp.beta_   # zero residuals -> b floored
+    assert gp._estimate_b(np.random.default_rng(0)) == 1e-8
+
+
+# ---------------------------------------------------------------- joint prediction
+def _dense_vecchia_joint(gp, Xs, m, n_sim, seed):
+    """The joint predictive law of the observed-first Vecchia approximation
+    written from its definition: prediction point i (in the maximin order of
+    the prediction points) regresses on its min(m, n) nearest predecessors
+    among all observations and the earlier prediction points,
+    y_i = b_i' y_N(i) + e_i, e_i ~ N(0, d_i), with observation neighbours noisy
+    and prediction points noise-free (gp.py:321-339).  Then
+    y_p = B_pp y_p + B_po y_o + e, so mean = (I - B_pp)^-1 B_po r_o and
+    cov = (I - B_pp)^-1 D (I - B_pp)^-T; paths use the same standard normal
+    draws as predict_joint (gp.py:389-397)."""
+    n, d = gp.X_.shape
+    ns = len(Xs)
+    th = gp.theta_
+    lam, var, nug = np.exp(th[1:1 + d]), math.exp(th[0]), math.exp(th[-1])
+    Xp = Xs / lam
+    first = int(np.argmin(((Xp - Xp.mean(0)) ** 2).sum(1)))
+    porder = _maximin_bruteforce(Xp, first)
+    Xall = np.vstack([gp.X_, Xs[porder]])      # observations, then ordered predictions
+    noisy = np.r_[np.ones(n), np.zeros(ns)]
+    k = min(m, n)
+    B = np.zeros((ns, n + ns))
+    D = np.empty(ns)
+    for i in range(ns):
+        row = n + i
+        dist = np.linalg.norm((Xall[:row] - Xall[row]) / lam, axis=1)
+        nb = np.argsort(dist, kind="stable")[:k]
+        P = np.vstack([Xall[nb], Xall[row]])
+        C = var * O.matern_corr(scipy.spatial.distance.cdist(P / lam, P / lam), 2.5)
+        C[np.arange(k), np.arange(k)] += var * nug * noisy[nb]
+        w = np.linalg.solve(C[:k, :k], C[:k, k])
+        B[i, nb] = w
+        D[i] = C[k, k] - w @ C[:k, k]
+    resid = gp.y_ - O.design(gp.X_, gp.trend) @ gp.beta_
+    IB = np.eye(ns) - B[:, n:]
+    mu_ord = np.linalg.solve(IB, B[:, :n] @ resid)
+    cov = np.linalg.solve(IB, np.linalg.solve(IB, np.diag(D)).T)
+    inv = np.empty(ns, dtype=np.int64)
+    inv[porder] = np.arange(ns)
+    mu = mu_ord[inv] + O.design(Xs, gp.trend) @ gp.beta_
+    eps = np.random.default_rng(seed).standard_normal((ns, n_sim))
+    dr = np.linalg.solve(IB, np.sqrt(D)[:, None] * eps)[inv]
+    samples = (mu[:, None] + math.sqrt(gp.b_) * dr).T
+    return (gp._ymu + gp._ysd * mu, gp._ysd ** 2 * gp.b_ * np.diag(cov)[inv],
+            gp._ymu + gp._ysd * samples)
+
+
+@pytest.mark.parametrize("m", [6, 80])
+def test_predict_joint_matches_dense_vecchia_definition(m):
+    """Several prediction points: the prediction-point maximin order
+    (gp.py:322), the neighbour sets among observations and earlier prediction
+    points (gp.py:326, capped at min(m, n)), the noise-free mask of prediction
+    neighbours (gp.py:339) and the three triangular solves (gp.py:380-397)
+    against the dense definition.  m = 80 >= n + ns - 1 conditions every
+    prediction point on n of its predecessors (the cap keeps it from all)."""
+    X, y = _sine(60, seed=12)
+    gp = O.OracleGP(m_est=10, m_pred=20, n_est=60, trend="constant", nugget=1e-3,
+                    random_state=4).fit(X, y)
+    Xt = np.random.default_rng(13).random((14, 2))
+    out = gp.predict_joint(Xt, m=m, exact_var=True, n_sim=7, random_state=21)
+    Xs = (Xt - gp._lo) / gp._span
+    mean, var, samples = _dense_vecchia_joint(gp, Xs, m, 7, 21)
+    np.testing.assert_allclose(out["mean"], mean, rtol=1e-9, atol=1e-11)
+    np.testing.assert_allclose(out["var"], var, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(out["samples"], samples, rtol=1e-9, atol=1e-10)
