@@ -20,6 +20,7 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.dont_write_bytecode = True      # the problems module is always read from source
 sys.path.insert(0, HERE)
 import fs_problems  # noqa: E402
 
