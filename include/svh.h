@@ -141,6 +141,22 @@ int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t nt
                 double nu, int32_t precision, int32_t noise_free, double* mean,
                 double* var, int64_t* nb_out);
 
+/* svh_predict without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Every array is a
+ * DEVICE pointer of this context's GPU: Xtr (ntr x d), resid (ntr), Xq
+ * (nq x d) in, mean and var (nq) out; theta stays on the host.  Same results
+ * as svh_predict (neighbour indices are not returned).  Working memory comes
+ * from a fixed 64 MiB device array that the library holds on each GPU for the
+ * life of the process: 8 (ntr + nq) d + 8 nq m bytes (+ 64 KiB) must fit,
+ * otherwise SVH_ERR_ARG (split the queries).  Calls on one GPU share that
+ * array: calls that overlap in time on one GPU (other threads, other
+ * contexts) race.  Synchronous on return. */
+int svh_predict_noalloc(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t ntr,
+                        const double* Xq, int64_t nq, int32_t d, int32_t m, const double* theta,
+                        double nu, int32_t precision, int32_t noise_free, double* mean,
+                        double* var);
+
 /* Columns of the sparse inverse Cholesky factor for arbitrary blocks
  * (gp.py:332-348, _joint_factor): block b is rows blocks[b*K .. b*K+K-1] of
  * Xall (self last), nugget mask mask[b*K + j] (NULL = all ones); writes

@@ -63,6 +63,9 @@ def _load():
     L.svh_predict.argtypes = [_vp, _dp, _dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int32,
                               ctypes.c_int32, _dp, ctypes.c_double, ctypes.c_int32,
                               ctypes.c_int32, _dp, _dp, _i64p]
+    L.svh_predict_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
+                                      ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_double,
+                                      ctypes.c_int32, ctypes.c_int32, _vp, _vp]
     L.svh_block_columns.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, _i64p, _dp,
                                     ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_double,
                                     ctypes.c_int32, _dp]
@@ -108,7 +111,8 @@ def _load():
                  "svh_batch_rebuild", "svh_joint_solve", "svh_batch_fit_start", "svh_batch_fit_step",
                  "svh_batch_get", "svh_batch_set_reorder", "svh_batch_eval",
                  "svh_timing_enable", "svh_timing_read", "svh_timing_reset", "svh_maximin", "svh_ordered_knn", "svh_ordered_knn_rows", "svh_knn_query",
-                 "svh_vecchia_setup", "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_block_columns", "svh_create"):
+                 "svh_vecchia_setup", "svh_vecchia_rebuild", "svh_vecchia_eval", "svh_predict", "svh_predict_noalloc", "svh_block_columns",
+                 "svh_create"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -123,7 +127,7 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
             "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild",
-            "svh_joint_solve", "svh_ordered_knn_rows")
+            "svh_joint_solve", "svh_ordered_knn_rows", "svh_predict_noalloc")
 
 
 def device_count() -> int:
@@ -331,6 +335,15 @@ class Context:
             _ptr(theta), float(nu), PRECISION[precision], int(bool(noise_free)), _ptr(mean),
             _ptr(var), _ptr(nb, _i64p)))
         return (mean, var, nb) if return_nb else (mean, var)
+
+    def predict_noalloc(self, Xtr, resid, ntr, Xq, nq, d, m, theta, nu, mean, var,
+                        noise_free=True, precision="fp64"):
+        """svh_predict_noalloc: Xtr, resid, Xq, mean, var are DEVICE addresses
+        (ints) on this context's GPU; theta is a host array."""
+        theta = _finite(theta, "theta")
+        self._check(self._L.svh_predict_noalloc(
+            self._h, Xtr, resid, int(ntr), Xq, int(nq), int(d), int(m), _ptr(theta), float(nu),
+            PRECISION[precision], int(bool(noise_free)), mean, var))
 
     def block_columns(self, Xall, blocks, mask, theta, nu, precision="fp64"):
         Xall, theta = _finite(Xall, "X"), _finite(theta, "theta")

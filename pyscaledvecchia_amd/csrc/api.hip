@@ -3,6 +3,8 @@
 // include/svh.h.
 
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "svh_internal.h"
@@ -44,6 +46,31 @@ void batch_eval_device(svh_ctx* ctx, const double* theta, double* ll, double* gr
                        double* fisher, double* beta, int32_t* jitter_rounds);
 
 namespace {
+
+// Fixed working array of svh_predict_noalloc: a device global, so it exists on
+// every GPU the library is loaded on (each GPU its own copy) for the life of
+// the process.  Host code must not pass `scratch` itself to a kernel or copy:
+// on the host that name is only a stand-in; the device address comes from
+// hipGetSymbolAddress (scratch_address) and is kept per device.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) char scratch[kScratchBytes];
+
+// Device address of this device's `scratch`, looked up once per device with
+// that device current (called from svh_create).  nullptr if the lookup fails.
+void* scratch_address(int device) {
+  static std::mutex mu;
+  static std::unordered_map<int, void*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+  }
+  addr[device] = p;
+  return p;
+}
 
 template <class F>
 int guarded(svh_ctx* ctx, F&& f) {
@@ -101,6 +128,7 @@ int svh_create(int device, svh_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  ctx->scratch = scratch_address(device);
   *out = ctx;
   return SVH_OK;
 }
@@ -271,6 +299,62 @@ int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t nt
     download(ctx, mean, dmean, (size_t)nq * sizeof(double));
     download(ctx, var, dvar, (size_t)nq * sizeof(double));
     if (nb_out) download(ctx, nb_out, dnb, (size_t)nq * m * sizeof(int64_t));
+    sync(ctx);
+  });
+}
+
+int svh_predict_noalloc(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t ntr,
+                        const double* Xq, int64_t nq, int32_t d, int32_t m, const double* theta,
+                        double nu, int32_t precision, int32_t noise_free, double* mean,
+                        double* var) {
+  return guarded(ctx, [&] {
+    SVH_REQUIRE(ntr >= 1 && nq >= 0 && d >= 1 && d <= 32, "predict_noalloc: bad sizes");
+    SVH_REQUIRE(m >= 1 && m <= ntr, "predict_noalloc: need 1 <= m <= ntr");
+    SVH_REQUIRE(precision == SVH_FP64 || precision == SVH_MIXED,
+                "predict_noalloc: bad precision mode");
+    SVH_REQUIRE(Xtr && resid && theta && mean && var, "predict_noalloc: null pointer");
+    SVH_REQUIRE(ctx->scratch, "predict_noalloc: the fixed device array is not available");
+    // working memory: SoA copies of both point sets, the neighbour lists, and
+    // at most 64 KiB of small per-call buffers (ranges, jitter levels/flags)
+    const double need = 8.0 * ((double)(ntr + nq) * d + (double)nq * m) + 65536.0;
+    SVH_REQUIRE(need <= (double)kScratchBytes,
+                "predict_noalloc: 8 (ntr + nq) d + 8 nq m bytes exceed the 64 MiB fixed device "
+                "array; split the queries");
+    if (nq == 0) return;
+    // every request of this call is carved from `scratch` (no hipMalloc), the
+    // host staging is a small array of the context, timers and tuner are off
+    struct Fixed {
+      svh_ctx* c;
+      bool timers;
+      int tune;
+      Fixed(svh_ctx* c_) : c(c_), timers(c_->timers.enabled), tune(c_->tune.mode) {
+        c->arena.fix_base = static_cast<char*>(c->scratch);
+        c->arena.fix_bytes = kScratchBytes;
+        c->arena.fix_used = 0;
+        c->stage.fixed = true;
+        c->timers.enabled = false;
+        c->tune.mode = 0;
+      }
+      ~Fixed() {
+        c->arena.fix_base = nullptr;
+        c->arena.fix_bytes = c->arena.fix_used = 0;
+        c->stage.fixed = false;
+        c->timers.enabled = timers;
+        c->tune.mode = tune;
+      }
+    } fixed(ctx);
+    double* dRT = ctx->arena.get_t<double>("pr.RT", (size_t)ntr * d);
+    double* dQT = ctx->arena.get_t<double>("pr.QT", (size_t)nq * d);
+    int64_t* dnb = ctx->arena.get_t<int64_t>("pr.nb", (size_t)nq * m);
+    double* dlam = ctx->arena.get_t<double>("pr.lam", 32);
+    double lam[32];
+    for (int l = 0; l < d; ++l) lam[l] = std::exp(theta[1 + l]);
+    upload(ctx, dlam, lam, d * sizeof(double));
+    launch_to_soa(ctx, Xtr, ntr, d, dlam, dRT);
+    launch_to_soa(ctx, Xq, nq, d, dlam, dQT);
+    launch_knn_query(ctx, dRT, ntr, dQT, nq, d, m, dnb, m);
+    predict_blocks_device(ctx, Xtr, dnb, m, Xq, nullptr, 0, nullptr, noise_free, nq, m + 1, d,
+                          theta, nu, resid, 1, nullptr, mean, var, nullptr, precision);
     sync(ctx);
   });
 }

@@ -32,13 +32,28 @@ struct SvhError {
 // ---------------------------------------------------------------- buffers
 // Named device buffers that only grow: repeated calls of one size allocate
 // once.  Pointers stay valid until the same name is requested larger.
+//
+// Fixed-region mode (the entry point that allocates nothing): while a region
+// is set, every request is carved from it in call order (256-byte aligned,
+// nothing cached, no hipMalloc); a request that does not fit throws
+// SVH_ERR_ARG.
 struct DevArena {
   struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
   };
   std::unordered_map<std::string, Buf> bufs;
+  char* fix_base = nullptr;
+  size_t fix_bytes = 0, fix_used = 0;
   void* get(const std::string& name, size_t bytes) {
+    if (fix_base) {
+      const size_t at = (fix_used + 255) & ~size_t(255);
+      if (bytes > fix_bytes || at > fix_bytes - bytes)
+        throw SvhError{SVH_ERR_ARG, "working memory of the call exceeds the fixed device "
+                                    "array (request '" + name + "')"};
+      fix_used = at + bytes;
+      return fix_base + at;
+    }
     Buf& b = bufs[name];
     if (bytes == 0) bytes = 1;
     if (b.bytes < bytes) {
@@ -60,11 +75,18 @@ struct DevArena {
   }
 };
 
-// Pinned host staging for small results.
+// Pinned host staging for small results.  In fixed mode (no allocation) a
+// small pageable array of the context serves instead.
 struct HostStage {
   void* p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
+  alignas(64) char fixed_buf[256];
   void* get(size_t want) {
+    if (fixed) {
+      if (want > sizeof fixed_buf) throw SvhError{SVH_ERR_ARG, "host staging too small"};
+      return fixed_buf;
+    }
     if (bytes < want) {
       if (p) SVH_HIP(hipHostFree(p));
       p = nullptr;
@@ -178,6 +200,7 @@ struct svh_ctx {
   bool batch_data = false;   // unordered batch data uploaded for the current batch
   MlpState* mlp = nullptr;
   Autotune tune;
+  void* scratch = nullptr;   // this device's fixed working array (api.hip), or nullptr
   std::string last_error;
 };
 
