@@ -87,6 +87,7 @@ struct LLArgs {
   double var, nug;
   int rec_stride;  // doubles between consecutive block records
   unsigned long long* prof;  // per-phase shader-clock sums (profiling runs), or NULL
+  int zmode;       // trend columns: 0 gathered from Z, 1 Z = [1, x], 2 Z = 1
 };
 
 __device__ __forceinline__ void matern(int nu_code, double r, double& M, double& g) {
@@ -887,16 +888,30 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   const bool act = bvalid && r < K;
   int pt = 0;
   if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
-  double x[DM];
+  double x[DM], raw[DM];
 #pragma unroll
   for (int l = 0; l < DM; ++l) {
-    x[l] = (act && l < d) ? a.X[(int64_t)pt * d + l] * a.inv_lam[l] : 0.0;
+    raw[l] = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
+    x[l] = (l < d) ? raw[l] * a.inv_lam[l] : 0.0;
     xs[r * DM + l] = x[l];
   }
   double bv[1 + kRQ];
   bv[0] = act ? a.y[pt] : 0.0;
+  // trend columns: gathered, or formed from the inputs (Z = [1, x] or Z = 1)
+  const int zmode = a.zmode;
 #pragma unroll
-  for (int c = 0; c < kRQ; ++c) bv[1 + c] = (act && c < q) ? a.Z[(int64_t)pt * q + c] : 0.0;
+  for (int c = 0; c < kRQ; ++c) {
+    double z = 0.0;
+    if (act && c < q) {
+      if (zmode == 0)
+        z = a.Z[(int64_t)pt * q + c];
+      else if (c == 0)
+        z = 1.0;
+      else
+        z = raw[c - 1 < DM ? c - 1 : 0];
+    }
+    bv[1 + c] = z;
+  }
   const double var = a.var, nug = a.nug;
   double jit = 0.0;
   if (lev > 0) {
@@ -1450,6 +1465,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   a.prof = nullptr;
+  a.zmode = 0;
   for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
     split_block<-1, false, false>(a, b, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
@@ -1497,6 +1513,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
   a.nug = B.par[i * 16 + 1];
   a.rec_stride = kBNC;
   a.prof = nullptr;
+  a.zmode = 0;
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1877,6 +1894,26 @@ static void int_fill(svh_ctx* ctx, int* p, int v, int64_t n) {
   SVH_HIP(hipGetLastError());
 }
 
+// Trend columns the likelihood kernels can form from the inputs instead of
+// gathering them: 1 for Z = [1, X] exactly (linear trend), 2 for Z = 1
+// (constant trend), 0 otherwise.
+static int trend_mode(const double* X, const double* Z, int64_t n, int d, int q) {
+  if (q == 1) {
+    for (int64_t i = 0; i < n; ++i)
+      if (!(Z[i] == 1.0)) return 0;
+    return 2;
+  }
+  if (q == d + 1) {
+    for (int64_t i = 0; i < n; ++i) {
+      if (!(Z[i * q] == 1.0)) return 0;
+      for (int l = 0; l < d; ++l)
+        if (!(Z[i * q + 1 + l] == X[i * d + l])) return 0;
+    }
+    return 1;
+  }
+  return 0;
+}
+
 void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, const double* Zo,
                           int64_t n, int d, int q, int64_t nblk, const int64_t* rows,
                           const int64_t* nbr, const int32_t* kcnt, int mmax,
@@ -1925,6 +1962,7 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
   vs.kp = (mmax + 1 <= 32) ? 32 : (mmax + 1 <= 64 ? 64 : 128);
   vs.nchunk_grad = nchunk_grad;
   vs.nchunk_val = nchunk_val;
+  vs.zmode = q > 0 ? trend_mode(Xo, Zo, n, d, q) : 0;
   vs.ready = true;
 }
 
@@ -1978,6 +2016,7 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
     ctx->rb_n = n;
     ctx->rb_d = d;
     ctx->rb_q = q;
+    ctx->rb_zmode = q > 0 ? trend_mode(Xe, Ze, n, d, q) : 0;
   }
   SVH_REQUIRE(ctx->rb_n == n && ctx->rb_d == d && ctx->rb_q == q,
               "rebuild: data not uploaded for these sizes");
@@ -2019,6 +2058,7 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   vs.kp = (m + 1 <= 32) ? 32 : (m + 1 <= 64 ? 64 : 128);
   vs.nchunk_grad = (int)(m + (rest + stepg - 1) / stepg);
   vs.nchunk_val = (int)(m + (rest + stepv - 1) / stepv);
+  vs.zmode = ctx->rb_zmode;
   vs.ready = true;
 }
 
@@ -2131,6 +2171,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.nug = std::exp(theta[1 + d]);
   a.rec_stride = NC;
   a.prof = nullptr;
+  a.zmode = vs.zmode;
   const char* penv = std::getenv("SVH_PHASE_PROFILE");
   if (penv && (use_split || use_rows)) {
     a.prof = ctx->arena.get_t<unsigned long long>("v.prof", 8);

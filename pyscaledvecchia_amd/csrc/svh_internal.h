@@ -108,6 +108,7 @@ struct VecchiaState {
   int64_t n = 0, nblk = 0;
   int d = 0, q = 0, mmax = 0, kp = 0;
   int nchunk_grad = 0, nchunk_val = 0;
+  int zmode = 0;   // trend columns formed in the kernels: 0 none, 1 [1, x], 2 constant
   // device pointers live in the arena under "v.*" names
 };
 
@@ -196,7 +197,7 @@ struct svh_ctx {
   KernelTimers timers;
   int mm_rounds = 0;  // rounds of the last maximin call
   int64_t rb_n = 0;   // sizes of the estimation data resident for rebuilds
-  int rb_d = 0, rb_q = 0;
+  int rb_d = 0, rb_q = 0, rb_zmode = 0;
   bool batch_data = false;   // unordered batch data uploaded for the current batch
   MlpState* mlp = nullptr;
   Autotune tune;
