@@ -132,6 +132,72 @@ def run_c4(args):
 PEAK_FP64_MFMA_TFLOPS = 78.6
 
 
+def run_mixed(args):
+    """Mixed precision against FP64 per nugget decade: C2 inputs (n = 50,000,
+    d = 8) with y = the sin-sum signal + N(0, tau var(signal)), tau = 1e-8 ..
+    1e-2.  For each tau: one objective evaluation in both modes at the FP64
+    estimate on one structure (loglik, gradient, Fisher, beta errors), whole
+    fits in both modes (estimates compared raw and re-evaluated on one FP64
+    structure), predictive mean and sd at 2000 test points, fit and pass-1
+    times."""
+    from pyscaledvecchia_amd import ScaledVecchiaGP, _lib
+    ctx = _lib.context()
+    n, d, m = args.n, 8, 30
+    rng = np.random.default_rng(1002)
+    X = rng.random((n, d))
+    f = (A_COEF * np.sin(2 * np.pi * X / ELL)).sum(1) + 0.1 * X[:, 0] * X[:, 1]
+    z = rng.standard_normal(n)
+    Xt = np.random.default_rng(1012).random((2000, d))
+    kw = dict(m_est=m, m_pred=140, nu=2.5, trend="linear", n_est=n)
+
+    def rel(a, b):
+        return float(np.max(np.abs(np.asarray(a) - b)) / np.max(np.abs(b)))
+    ScaledVecchiaGP(**kw).fit(X, f + 0.01 * z)          # warm-up
+    rows = []
+    for tau in [1e-8, 1e-7, 1e-6, 1e-5, 1e-4, 1e-3, 1e-2]:
+        y = f + math.sqrt(tau * f.var()) * z
+        fits, times, p1 = {}, {}, {}
+        for prec in ("fp64", "mixed", "mixed_f32cov"):
+            ctx.timing(True)
+            ctx.timing_reset()
+            t0 = time.perf_counter()
+            fits[prec] = ScaledVecchiaGP(precision=prec, **kw).fit(X, y)
+            times[prec] = time.perf_counter() - t0
+            ctx.timing(False)
+            ms, cnt = ctx.timing_read("vecchia_pass1")
+            p1[prec] = ms / max(cnt, 1)
+        g6 = fits["fp64"]
+        # one FP64 structure: the one resident after the last fit
+        st = fits["mixed_f32cov"]._fit_state["struct"]
+        lvt = math.log(max(g6.y_.var(), 1e-8))
+        e64 = st(g6.theta_, 2.5, var_penalty=1.0, log_var_target=lvt)
+        mu6, sd6 = g6.predict(Xt, return_std=True)
+        row = {"tau": tau, "fitted_nugget": g6.nugget_, "fit_s": times, "pass1_ms": p1,
+               "evals": {k: v.n_evals_ for k, v in fits.items()}}
+        for prec in ("mixed", "mixed_f32cov"):
+            gm = fits[prec]
+            emx = st(g6.theta_, 2.5, var_penalty=1.0, log_var_target=lvt, precision=prec)
+            ll_m_common = st(gm.theta_, 2.5, var_penalty=1.0, log_var_target=lvt)[0]
+            mum, sdm = gm.predict(Xt, return_std=True)
+            row[prec] = {
+                "eval_rel_err": {"loglik": abs(emx[0] - e64[0]) / abs(e64[0]),
+                                 "grad": rel(emx[1], e64[1]), "fisher": rel(emx[2], e64[2]),
+                                 "beta": rel(emx[3], e64[3])},
+                "fit_rel_err": {"loglik_raw": abs(gm.loglik_ - g6.loglik_) / abs(g6.loglik_),
+                                "loglik_common_structure":
+                                    abs(ll_m_common - e64[0]) / abs(e64[0]),
+                                "theta": rel(gm.theta_, g6.theta_), "pred_mean": rel(mum, mu6),
+                                "pred_sd": rel(sdm, sd6)},
+                "refine_iters": gm.n_refine_}
+        rows.append(row)
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "mixed precision vs FP64 per nugget decade (C2 inputs)",
+                      "unit": "relative error", "n_gpus": 1, "dtype": "f32+f64 vs f64",
+                      "data": "synthetic (sin-sum signal, noise tau var)",
+                      "config": {"workload": f"C2 inputs n={n} d={d} m_est={m} nu=2.5 linear"},
+                      "rows": rows}))
+
+
 def c5_inputs(n, ns, d=10, seed=1005):
     """Prediction workload after the large-prediction configuration: dims 1-3
     from a two-component Gaussian mixture (means 0.3 / 0.7, sd 0.1, clipped
@@ -313,7 +379,8 @@ def main():
     ap.add_argument("--no-fit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5p", "c5j"])
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c3", "c4", "c5p", "c5j", "mixed"])
     ap.add_argument("--n-train", type=int, default=200_000)
     ap.add_argument("--n-test", type=int, default=100_000)
     ap.add_argument("--n-c4", type=int, default=50_000)
@@ -332,6 +399,9 @@ def main():
         return
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "mixed":
+        run_mixed(args)
         return
     if args.workload == "c5j":
         if args.n_test == 100_000:

@@ -28,8 +28,13 @@ extern "C" {
 
 /* precision modes of svh_vecchia_eval / svh_predict */
 #define SVH_FP64 0  /* every stage in float64                                  */
-#define SVH_MIXED 1 /* float32 derivative products / W solves / Fisher;
-                       float64 covariance, Cholesky, log-det, GLS              */
+#define SVH_MIXED 1 /* likelihood: float32 derivative terms dS_p a, W solves and
+                       Fisher / cross products (float32 accumulation); float64
+                       covariance, Cholesky, log-det, a = L^-T e_K, GLS.
+                       prediction blocks: float32 matrix-core trailing updates */
+#define SVH_MIXED_F32COV 2 /* svh_vecchia_eval only: SVH_MIXED plus the
+                       off-diagonal covariance entries (distances, Matern)
+                       assembled in float32; diagonal and factorisation float64 */
 
 typedef struct svh_ctx svh_ctx;
 

@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsvh.so")
 
 SVH_OK, SVH_ERR_CHOLESKY, SVH_ERR_ARG, SVH_ERR_HIP = 0, 1, 2, 3
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_f32cov": 2}
 
 _lib = None
 _lock = threading.Lock()

@@ -22,9 +22,12 @@
 // written as G [a | a x~ | a x~^2] with coordinates centred on the block's
 // own point, and the cross products W^T [W | L^-1 y | L^-1 Z] behind the
 // Fisher information and the gradient records.  FP64 mode runs them as FP64
-// MFMA; mixed mode (SVH_MIXED) as FP32 MFMA with FP32 accumulation.  The
-// covariance, Cholesky, log-determinant, triangular solves and GLS are FP64
-// in both modes, so the log-likelihood and beta do not depend on the mode.
+// MFMA; the mixed modes as FP32 MFMA with FP32 accumulation.  In the rows
+// kernel the mixed modes also compute dS_p a (distances, g(r)) and the solve
+// W = L^-1 (dS a) in FP32; SVH_MIXED_F32COV in addition assembles the
+// off-diagonal covariance entries in FP32.  The diagonal, the Cholesky,
+// log-determinant, a = L^-T e_K, L^-1 [y Z] and the GLS solve are FP64 in
+// every mode.
 //
 // Cached record per block (need_grad):  u, v[q], log L_KK, wK[P], s[P], T[P*q]
 // where u = (L^-1 y)_K, v = (L^-1 Z)_K, W_p = L^-1 dS_p a, s_p = W_p.(L^-1 y),
@@ -88,6 +91,7 @@ struct LLArgs {
   int rec_stride;  // doubles between consecutive block records
   unsigned long long* prof;  // per-phase shader-clock sums (profiling runs), or NULL
   int zmode;       // trend columns: 0 gathered from Z, 1 Z = [1, x], 2 Z = 1
+  int sig32;       // mixed kernels: off-diagonal covariance entries in float32
 };
 
 __device__ __forceinline__ void matern(int nu_code, double r, double& M, double& g) {
@@ -370,6 +374,39 @@ __device__ __forceinline__ void matern_t(double r, double& M, double& g, int nu_
     double e = exp(-c);
     M = (1.0 + c + c * c * (1.0 / 3.0)) * e;
     g = -(5.0 / 3.0) * (1.0 + c) * e;
+  }
+}
+
+// Matern M and g = M'/r in float32 (the mixed mode's covariance assembly and
+// derivative terms): the same closed forms, hardware exp2 and sqrt.
+template <int NU>
+__device__ __forceinline__ void matern_f(float r, float& M, float& g, int nu_rt = 2) {
+  if constexpr (NU < 0) {
+    if (nu_rt == 0)
+      matern_f<0>(r, M, g);
+    else if (nu_rt == 1)
+      matern_f<1>(r, M, g);
+    else
+      matern_f<2>(r, M, g);
+  } else if constexpr (NU == 0) {
+    if (r > 0.0f) {
+      const float e = __expf(-r);
+      M = e;
+      g = -e / r;
+    } else {
+      M = 1.0f;
+      g = 0.0f;
+    }
+  } else if constexpr (NU == 1) {
+    const float c = 1.7320508f * r;
+    const float e = __expf(-c);
+    M = (1.0f + c) * e;
+    g = -3.0f * e;
+  } else {
+    const float c = 2.2360680f * r;
+    const float e = __expf(-c);
+    M = (1.0f + c + c * c * (1.0f / 3.0f)) * e;
+    g = -(5.0f / 3.0f) * (1.0f + c) * e;
   }
 }
 
@@ -836,6 +873,17 @@ __device__ __forceinline__ double half_bcast(double v) {
   return __longlong_as_double(((long long)sh[o] << 32) | sl[o]);
 }
 
+// float v of lane SRC of this lane's half (the 32-bit form of half_bcast).
+template <int SRC>
+__device__ __forceinline__ float half_bcast_f(float v) {
+  static_assert(SRC >= 0 && SRC < 32, "source lane of a half");
+  const int t = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + (SRC & 15), 0xF, 0xF,
+                                            false);
+  const auto sw = __builtin_amdgcn_permlane16_swap((unsigned)t, (unsigned)t, false, false);
+  constexpr int o = SRC < 16 ? 0 : 1;
+  return __int_as_float((int)sw[o]);
+}
+
 // LDS doubles per wave of the rows kernel: per half [cb 32 | bb 16 | ab 32 |
 // (unused) 16 | xs 32*dm] behind one base, then misc (16) and Mt (24 x 33)
 __host__ __device__ inline int rows_half_doubles(int dm) { return 96 + 32 * dm; }
@@ -862,6 +910,25 @@ __device__ __forceinline__ void w_sweep(double (&R)[kRC], const double (&S)[32],
     __builtin_amdgcn_sched_barrier(0);
   }
   if constexpr (K + 1 < 31) w_sweep<K + 1>(R, S, r);
+}
+
+// w_sweep in float32 (mixed mode): the unit factor's entries rounded to float,
+// float right-hand sides and float accumulation.
+template <int K>
+__device__ __forceinline__ void w_sweep_f(float (&R)[kRC], const double (&S)[32], int r) {
+  const float l = (r > K) ? (float)S[K] : 0.0f;
+#pragma unroll
+  for (int c0 = 0; c0 < kRC; c0 += 4) {
+    float wk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wk[u] = half_bcast_f<K>(R[c0 + u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) R[c0 + u] = fmaf(-l, wk[u], R[c0 + u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(R[c0 + u]));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if constexpr (K + 1 < 31) w_sweep_f<K + 1>(R, S, r);
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
@@ -929,15 +996,29 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     if (c == r) {
       v = act ? var * (1.0 + nug) + jit : 1.0;
     } else if (c < r && act) {
-      double s2 = 0.0;
+      if (MIX && a.sig32) {
+        // float32 assembly of the off-diagonal entries (coordinate
+        // differences taken in float64, then rounded)
+        float s2 = 0.0f;
 #pragma unroll
-      for (int l = 0; l < DM; ++l) {
-        const double t = x[l] - xs[c * DM + l];
-        s2 = fma(t, t, s2);
+        for (int l = 0; l < DM; ++l) {
+          const float t = (float)(x[l] - xs[c * DM + l]);
+          s2 = fmaf(t, t, s2);
+        }
+        float M, g;
+        matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
+        v = var * (double)M;
+      } else {
+        double s2 = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          const double t = x[l] - xs[c * DM + l];
+          s2 = fma(t, t, s2);
+        }
+        double M, g;
+        matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+        v = var * M;
       }
-      double M, g;
-      matern_t<NU>(sqrt(s2), M, g, a.nu_code);
-      v = var * M;
     }
     S[c] = v;
     __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
@@ -1004,7 +1085,30 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     wave_sync();
     SVH_PHASE(3)
     // ---- dS_p a, row r
-    if (act) {
+    if (act && MIX) {
+      // float32 terms and accumulation
+      float Rf[DM];
+#pragma unroll
+      for (int l = 0; l < DM; ++l) Rf[l] = 0.0f;
+#pragma unroll 2
+      for (int c = 0; c < 32; ++c) {
+        const float ac = (float)ab[c];
+        float dl[DM], s2 = 0.0f;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          dl[l] = (float)(x[l] - xs[c * DM + l]);
+          s2 = fmaf(dl[l], dl[l], s2);
+        }
+        float M, g;
+        matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
+        const float w = g * ac;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) Rf[l] = fmaf(w, dl[l] * dl[l], Rf[l]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int l = 0; l < DM; ++l) R[1 + l] = -var * (double)Rf[l];
+    } else if (act) {
 #pragma unroll 2
       for (int c = 0; c < 32; ++c) {
         const double ac = ab[c];
@@ -1023,14 +1127,26 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       }
 #pragma unroll
       for (int l = 0; l < DM; ++l) R[1 + l] *= -var;
+    }
+    if (act) {
       R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
       R[kRC - 1] = var * nug * av;
     }
     SVH_PHASE(4)
     // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
-    w_sweep<0>(R, S, r);
+    if constexpr (MIX) {
+      float Rf[kRC];
 #pragma unroll
-    for (int c = 0; c < kRC; ++c) R[c] *= dinv;
+      for (int c = 0; c < kRC; ++c) Rf[c] = (float)R[c];
+      w_sweep_f<0>(Rf, S, r);
+      const float df = (float)dinv;
+#pragma unroll
+      for (int c = 0; c < kRC; ++c) R[c] = (double)(Rf[c] * df);
+    } else {
+      w_sweep<0>(R, S, r);
+#pragma unroll
+      for (int c = 0; c < kRC; ++c) R[c] *= dinv;
+    }
     SVH_PHASE(5)
   }
   // ---- products and records, one block at a time on the whole wave
@@ -1466,6 +1582,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   a.prof = nullptr;
   a.zmode = 0;
+  a.sig32 = 0;
   for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
     split_block<-1, false, false>(a, b, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
@@ -1514,6 +1631,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
   a.rec_stride = kBNC;
   a.prof = nullptr;
   a.zmode = 0;
+  a.sig32 = 0;
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2068,7 +2186,8 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   VecchiaState& vs = ctx->vs;
   SVH_REQUIRE(vs.ready, "eval: call svh_vecchia_setup first");
   SVH_REQUIRE(P == vs.d + 2, "eval: theta must have d+2 entries");
-  SVH_REQUIRE(precision == SVH_FP64 || precision == SVH_MIXED, "eval: bad precision mode");
+  SVH_REQUIRE(precision == SVH_FP64 || precision == SVH_MIXED || precision == SVH_MIXED_F32COV,
+              "eval: bad precision mode");
   int nu_code;
   if (nu == 0.5)
     nu_code = 0;
@@ -2172,6 +2291,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.rec_stride = NC;
   a.prof = nullptr;
   a.zmode = vs.zmode;
+  a.sig32 = precision == SVH_MIXED_F32COV ? 1 : 0;
   const char* penv = std::getenv("SVH_PHASE_PROFILE");
   if (penv && (use_split || use_rows)) {
     a.prof = ctx->arena.get_t<unsigned long long>("v.prof", 8);
@@ -2216,10 +2336,10 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
       }
     }
   };
-  Launch lau{s, &cands, nu_code, KP, lds_split, lds, &a, precision == SVH_MIXED};
+  Launch lau{s, &cands, nu_code, KP, lds_split, lds, &a, precision != SVH_FP64};
   // per-phase clocks exist in the split-row kernel and in the FP64 rows kernel
   // for nu = 5/2 with 5..8 dimensions
-  const bool rows_prof = use_rows && nu_code == 2 && d > 4 && d <= 8 && precision != SVH_MIXED;
+  const bool rows_prof = use_rows && nu_code == 2 && d > 4 && d <= 8 && precision == SVH_FP64;
   if (a.prof && !rows_prof) {
     for (int i = 0; i < (int)cands.size(); ++i)
       if (cands[i].kind == 1) dflt = i;

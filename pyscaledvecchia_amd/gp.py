@@ -41,10 +41,12 @@ class ScaledVecchiaGP:
     nugget         : None -> estimated relative nugget; float -> fixed
     var_correction : estimate the predictive-variance factor b (Sec. 3.4)
     lambda_max     : ranges above this are treated as infinite
-    precision      : 'fp64' (default) or 'mixed': FP32 matrix-core products for
-                     the derivative terms and Fisher information during the
-                     search (log-likelihood, Cholesky, GLS stay FP64), then
-                     FP64 refinement steps; predictions are FP64 either way
+    precision      : 'fp64' (default), 'mixed' or 'mixed_f32cov': during the
+                     search the derivative terms dS a, the W solves and the
+                     Fisher / cross products run in FP32 ('mixed_f32cov' also
+                     assembles the off-diagonal covariance entries in FP32);
+                     Cholesky, log-determinant, GLS stay FP64; then FP64
+                     refinement steps; predictions are FP64 either way
     theta0         : optional initial log-parameters for Fisher scoring
     warm_start     : None (default start), "learned" (the packaged network) or
                      a WarmStart instance: initial log-parameters predicted
@@ -304,7 +306,9 @@ class ScaledVecchiaGP:
                                               reorder_hook=reorder_hook, history=self.history_)
         self.n_iter_ = len(self.history_)
         self.n_refine_ = 0
-        if self.precision == "mixed":
+        if self.precision not in ("fp64", "mixed", "mixed_f32cov"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_f32cov'")
+        if self.precision != "fp64":
             # Refinement: the mixed search used FP64 log-likelihoods but FP32
             # derivative products, so its stationary point is that of a
             # slightly perturbed gradient.  Fisher scoring with FP64
