@@ -84,56 +84,48 @@ class BatchedFit:
                     if models[i].nugget is not None:
                         th[-1] = math.log(models[i].nugget)
                     preps[i]["theta"] = models[i]._clip_theta(th, preps[i]["d"])
-        mmax = max(p["m"] for p in preps)
+        # A learned start is judged by where its fit ends, not where it starts:
+        # each warm-started dataset is fitted twice in the same batch, from
+        # the learned start and from the reference start (gp.py:169-177), and
+        # the fit with the higher final log-likelihood is kept (the reference
+        # start on ties), so warm_start never returns a worse fit than the
+        # reference start would.
+        shadow = {i: nd + k for k, i in enumerate(ws_idx)}
+        preps_all = preps + [dict(preps[i], theta=preps[i]["theta_default"].copy())
+                             for i in ws_idx]
+        models_all = models + [models[i] for i in ws_idx]
+        na = len(preps_all)
+        mmax = max(p["m"] for p in preps_all)
         # the batch buffers are allocated empty (no ordered data, no
         # neighbour lists); every structure is then built on the device
-        d = np.array([p["d"] for p in preps], np.int32)
-        q = np.array([p["Ze"].shape[1] for p in preps], np.int32)
-        nu = np.array([_NU_CODE[g.nu] for g in models], np.int32)
-        n = np.array([len(p["ye"]) for p in preps], np.int64)
-        X = np.concatenate([np.ascontiguousarray(p["Xe"]).ravel() for p in preps])
-        y = np.concatenate([p["ye"] for p in preps])
-        Z = (np.concatenate([np.ascontiguousarray(p["Ze"]).ravel() for p in preps]) if q.sum()
-             else np.zeros(0))
-        chunks = [_chunk_ids_rows(len(p["ye"]), p["m"], p["d"] + 2) for p in preps]
+        d = np.array([p["d"] for p in preps_all], np.int32)
+        q = np.array([p["Ze"].shape[1] for p in preps_all], np.int32)
+        nu = np.array([_NU_CODE[g.nu] for g in models_all], np.int32)
+        n = np.array([len(p["ye"]) for p in preps_all], np.int64)
+        X = np.concatenate([np.ascontiguousarray(p["Xe"]).ravel() for p in preps_all])
+        y = np.concatenate([p["ye"] for p in preps_all])
+        Z = (np.concatenate([np.ascontiguousarray(p["Ze"]).ravel() for p in preps_all])
+             if q.sum() else np.zeros(0))
+        chunks = [_chunk_ids_rows(len(p["ye"]), p["m"], p["d"] + 2) for p in preps_all]
         chunk = np.concatenate([c for c, _ in chunks])
         nchunk = np.array([nc for _, nc in chunks], np.int32)
-        lvt = np.array([p["lvt"] for p in preps])
-        fixed = np.array([g.nugget is not None for g in models], np.int32)
-        lnug = np.array([math.log(g.nugget) if g.nugget is not None else 0.0 for g in models])
+        lvt = np.array([p["lvt"] for p in preps_all])
+        fixed = np.array([g.nugget is not None for g in models_all], np.int32)
+        lnug = np.array([math.log(g.nugget) if g.nugget is not None else 0.0
+                         for g in models_all])
         bh = _lib.BatchHandle(ctx)
         bh.setup(d, q, nu, n, None, None, None, None, None, chunk, nchunk, mmax, lvt, fixed,
                  lnug, models[0].lambda_max)
         bh.data(X, y, Z)
-        self._rebuild(bh, preps, list(range(nd)), [p["theta"] for p in preps])
+        self._rebuild(bh, preps_all, list(range(na)), [p["theta"] for p in preps_all])
         t_build = time.perf_counter() - t0
-        theta0 = np.zeros((nd, _lib.BP))
-        for i, p in enumerate(preps):
+        theta0 = np.zeros((na, _lib.BP))
+        for i, p in enumerate(preps_all):
             theta0[i, :p["d"] + 2] = p["theta"]
-        use_ws = np.zeros(nd, bool)
-        use_ws[ws_idx] = True
-        if ws_idx:
-            # start guard: keep a learned start only where its log-likelihood
-            # beats the default start's (one batched evaluation each)
-            thd = theta0.copy()
-            for i in ws_idx:
-                thd[i, :preps[i]["d"] + 2] = preps[i]["theta_default"]
-            try:
-                ll_ws = bh.eval(theta0)[0]
-                ll_def = bh.eval(thd)[0]
-                back = [i for i in ws_idx if not ll_ws[i] > ll_def[i]]
-            except np.linalg.LinAlgError:
-                back = list(ws_idx)
-            if back:
-                for i in back:
-                    preps[i]["theta"] = preps[i]["theta_default"].copy()
-                    theta0[i, :preps[i]["d"] + 2] = preps[i]["theta"]
-                    use_ws[i] = False
-                self._rebuild(bh, preps, back, [preps[i]["theta"] for i in back])
-        last = [np.exp(p["theta"][1:1 + p["d"]]).copy() for p in preps]
+        last = [np.exp(p["theta"][1:1 + p["d"]]).copy() for p in preps_all]
         bh.fit_start(theta0, models[0].max_iter, models[0].tol, models[0].var_penalty,
                      self.use_graph)
-        steps, rebuilds, t_reorder = 0, [1] * nd, 0.0
+        steps, rebuilds, t_reorder = 0, [1] * na, 0.0
         t0 = time.perf_counter()
         while True:
             st = bh.fit_step(1)
@@ -146,7 +138,7 @@ class BatchedFit:
                 th = bh.get()[0]
                 redo = []
                 for i in pend:
-                    g, p = models[i], preps[i]
+                    g, p = models_all[i], preps_all[i]
                     dd = p["d"]
                     thi = g._clip_theta(th[i, :dd + 2].copy(), dd)
                     r = np.exp(thi[1:1 + dd])
@@ -156,7 +148,8 @@ class BatchedFit:
                     redo.append(i)
                     last[i] = r.copy()
                 if redo:
-                    self._rebuild(bh, preps, redo, [th[i, :preps[i]["d"] + 2] for i in redo])
+                    self._rebuild(bh, preps_all, redo,
+                                  [th[i, :preps_all[i]["d"] + 2] for i in redo])
                     for i in redo:
                         bh.set_reorder(i, True)
                         rebuilds[i] += 1
@@ -165,32 +158,31 @@ class BatchedFit:
                 raise RuntimeError("batched Fisher scoring did not terminate")
         t_loop = time.perf_counter() - t0
         th, ll, beta, st, iters, nev = bh.get()
-        failed = [i for i in range(nd) if st[i] == _FAILED]
-        for i in failed:
-            if models[i].theta0 is not None or models[i].warm_start is None:
-                raise np.linalg.LinAlgError(f"dataset {i}: block Cholesky failed even with jitter")
-        if failed:
-            # learned starts that failed: refit those from the default start
-            redo = [models[i] for i in failed]
-            for g, i in zip(redo, failed):
-                g.theta0 = preps[i]["theta_default"].copy()
-            BatchedFit(redo, use_graph=self.use_graph, device=self.device).fit(
-                [Xs[i] for i in failed], [ys[i] for i in failed])
-            for g in redo:
-                g.theta0 = None
-                g.warm_start_used_ = False
+        ws_won = 0
         for i, (g, p) in enumerate(zip(models, preps)):
-            if st[i] == _FAILED:
-                continue
+            src = i
+            if i in shadow:
+                j = shadow[i]
+                learned_ok = st[i] != _FAILED
+                default_ok = st[j] != _FAILED
+                if not (learned_ok and (not default_ok or ll[i] > ll[j])):
+                    src = j
+                g.warm_start_used_ = src == i
+                ws_won += src == i
+                g.n_evals_both_ = int(nev[i]) + int(nev[j])
+            else:
+                g.warm_start_used_ = False
+            if st[src] == _FAILED:
+                raise np.linalg.LinAlgError(f"dataset {i}: block Cholesky failed even with jitter")
             dd, qq = p["d"], int(q[i])
-            g.n_evals_ = int(nev[i])
-            g.n_rebuilds_ = rebuilds[i]
-            g.n_iter_ = int(iters[i])
-            g.warm_start_used_ = bool(use_ws[i])
-            g._finish(th[i, :dd + 2], float(ll[i]), beta[i, :qq].copy(), p["rng"], p["n"])
+            g.n_evals_ = int(nev[src])
+            g.n_rebuilds_ = rebuilds[src]
+            g.n_iter_ = int(iters[src])
+            g._finish(th[src, :dd + 2], float(ll[src]), beta[src, :qq].copy(), p["rng"], p["n"])
         self.stats = dict(steps=steps, wall_s=time.perf_counter() - t_start,
                           build_s=t_build, loop_s=t_loop, reorder_s=t_reorder,
-                          evals=int(nev.sum()), rebuilds=int(sum(rebuilds)))
+                          evals=int(nev.sum()), rebuilds=int(sum(rebuilds)),
+                          warm_started=len(ws_idx), learned_kept=int(ws_won))
         return models
 
     @staticmethod

@@ -404,6 +404,10 @@ int svh_batch_setup(svh_ctx* ctx, int32_t ndata, const int32_t* d, const int32_t
     SVH_REQUIRE((X == nullptr) == (y == nullptr), "batch_setup: X and y both given or both NULL");
     SVH_REQUIRE((nbr == nullptr) == (kcnt == nullptr),
                 "batch_setup: nbr and kcnt both given or both NULL");
+    SVH_REQUIRE(ndata >= 1, "batch_setup: no datasets");
+    if (X && !Z)
+      for (int i = 0; i < ndata; ++i)
+        SVH_REQUIRE(q[i] == 0, "batch_setup: Z is NULL with q > 0");
     batch_setup_device(ctx, ndata, d, q, nu_code, n, X, y, Z, nbr, kcnt, chunk, nchunk, mmax, lvt,
                        fixed_nug, log_nug, lambda_max);
   });

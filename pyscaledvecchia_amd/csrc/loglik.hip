@@ -2428,6 +2428,7 @@ struct BatchState {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double var_penalty = 1.0;
+  double graph_var_penalty = 0.0;   // the value captured into `exec`
 };
 
 static BatchState& batch_of(svh_ctx* ctx) {
@@ -2710,7 +2711,16 @@ void batch_fit_start_device(svh_ctx* ctx, const double* theta0, int max_iter, do
                          ctx->stream));
   SVH_HIP(hipStreamSynchronize(ctx->stream));
   B.var_penalty = var_penalty;
+  // the state update takes var_penalty as a kernel argument: a graph captured
+  // with another value is captured again
+  if (B.exec && B.graph_var_penalty != var_penalty) {
+    SVH_HIP(hipGraphExecDestroy(B.exec));
+    SVH_HIP(hipGraphDestroy(B.graph));
+    B.exec = nullptr;
+    B.graph = nullptr;
+  }
   if (use_graph && !B.exec) {
+    B.graph_var_penalty = var_penalty;
     hipStream_t s = ctx->stream;
     SVH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     batch_enqueue(ctx, true);

@@ -49,9 +49,10 @@ class ScaledVecchiaGP:
                      refinement steps; predictions are FP64 either way
     theta0         : optional initial log-parameters for Fisher scoring
     warm_start     : None (default start), "learned" (the packaged network) or
-                     a WarmStart instance: initial log-parameters predicted
-                     from summary features of the data; a start whose
-                     Cholesky fails falls back to the default start
+                     a WarmStart instance: the data are also fitted from the
+                     initial log-parameters the network predicts from summary
+                     features, and that fit is kept only if it ends at a
+                     higher log-likelihood than the reference start's fit
     """
 
     m_est: int = 30
@@ -214,6 +215,8 @@ class ScaledVecchiaGP:
         return self
 
     def fit(self, X, y):
+        if self.theta0 is None and self.warm_start is not None:
+            return self._fit_learned(X, y)
         prep = self._prepare(X, y)
         n, d, rng = prep["n"], prep["d"], prep["rng"]
         Xe, ye, Ze, m, theta = prep["Xe"], prep["ye"], prep["Ze"], prep["m"], prep["theta"]
@@ -242,7 +245,6 @@ class ScaledVecchiaGP:
         rebuild(theta)
         last_ranges = np.exp(theta[1:1 + d]).copy()
         log_var_target = prep["lvt"]
-        use_ws = self.theta0 is None and self.warm_start is not None
 
         def objective(th, precision=self.precision):
             th = self._clip_theta(np.array(th, dtype=float), d)
@@ -270,40 +272,10 @@ class ScaledVecchiaGP:
             return True
 
         self.history_ = []
-        if use_ws:
-            # start guard: the learned start is kept only if its log-likelihood
-            # beats the default start's; otherwise the reference start is used
-            th_def = prep["theta_default"]
-            try:
-                ll_ws = objective(theta)[0]
-            except np.linalg.LinAlgError:
-                ll_ws = -np.inf
-            try:
-                ll_def = objective(th_def)[0]
-            except np.linalg.LinAlgError:
-                ll_def = -np.inf
-            if not ll_ws > ll_def:
-                use_ws = False
-                theta = th_def.copy()
-                rebuild(theta)
-                last_ranges = np.exp(theta[1:1 + d]).copy()
-        self.warm_start_used_ = use_ws
-        try:
-            theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
-                                              tol=self.tol, verbose=self.verbose,
-                                              reorder_hook=reorder_hook, history=self.history_)
-        except np.linalg.LinAlgError:
-            if not self.warm_start_used_:
-                raise
-            # the learned start failed: the default start is the fallback
-            self.warm_start_used_ = False
-            theta = prep["theta_default"].copy()
-            rebuild(theta)
-            last_ranges = np.exp(theta[1:1 + d]).copy()
-            self.history_ = []
-            theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
-                                              tol=self.tol, verbose=self.verbose,
-                                              reorder_hook=reorder_hook, history=self.history_)
+        self.warm_start_used_ = False
+        theta, ll, beta = _fisher_scoring(theta, objective, max_iter=self.max_iter,
+                                          tol=self.tol, verbose=self.verbose,
+                                          reorder_hook=reorder_hook, history=self.history_)
         self.n_iter_ = len(self.history_)
         self.n_refine_ = 0
         if self.precision not in ("fp64", "mixed", "mixed_f32cov"):
@@ -321,6 +293,35 @@ class ScaledVecchiaGP:
             self.n_refine_ = len(self.history_) - n0
         self._fit_state = state
         return self._finish(theta, ll, beta, rng, n)
+
+    def _fit_learned(self, X, y):
+        """Fit with the learned starting point, judged by where the fit ends:
+        the data are fitted from the learned start and from the reference
+        start (gp.py:169-177) and the fit with the higher final
+        log-likelihood is kept (the reference start on ties), so a learned
+        start never returns a worse fit than the reference would.  Both fits
+        share the batched fitter's launches when its limits allow (d <= 10,
+        trend columns <= 11, m_est <= 31); otherwise they run one after the
+        other."""
+        from dataclasses import replace
+        d = np.atleast_2d(X).shape[1]
+        if d <= 10 and self._design(np.zeros((1, d))).shape[1] <= 11 and self.m_est <= 31:
+            from .batch import BatchedFit
+            BatchedFit([self], device=self.device).fit([X], [y])
+            return self
+        prep = self._prepare(X, y)          # learned start from the network
+        try:
+            learned = replace(self, theta0=prep["theta"], warm_start=None).fit(X, y)
+        except np.linalg.LinAlgError:
+            learned = None          # the learned start's blocks fail: reference start
+        default = replace(self, theta0=None, warm_start=None).fit(X, y)
+        use = learned if learned is not None and learned.loglik_ > default.loglik_ else default
+        for k, v in vars(use).items():
+            if k not in ("theta0", "warm_start"):
+                setattr(self, k, v)
+        self.warm_start_used_ = use is learned
+        self.n_evals_both_ = default.n_evals_ + (learned.n_evals_ if learned is not None else 0)
+        return self
 
     # ---------------- prediction (gp.py:242-304) ----------------------------
     def _predict_scaled(self, Xs_std, X_tr, y_tr_resid, m, noise_free=True):

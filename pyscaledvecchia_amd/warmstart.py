@@ -344,7 +344,8 @@ def evaluate(ws, count=256, n=50_000, seed=7, batch=64, device=0, verbose=True):
             rec.append(dict(d=data[i][0].shape[1], nu=data[i][2],
                             iters_default=g0.n_iter_, iters_learned=g1.n_iter_,
                             evals_default=g0.n_evals_, evals_learned=g1.n_evals_,
-                            loglik_default=g0.loglik_, loglik_learned=g1.loglik_))
+                            loglik_default=g0.loglik_, loglik_learned=g1.loglik_,
+                            learned_kept=bool(g1.warm_start_used_)))
         if verbose:
             print(f"  {a + len(data)}/{count}: batch time default "
                   f"{runs['default'][1]:.2f} s, learned {runs['learned'][1]:.2f} s", flush=True)
@@ -366,7 +367,8 @@ def summarize(rec):
                 loglik_rel_gap_median=float(np.median(gap)),
                 loglik_rel_gap_min=float(gap.min()), loglik_rel_gap_max=float(gap.max()),
                 frac_within_1e6=float(np.mean(np.abs(gap) <= 1e-6)),
-                frac_learned_not_worse=float(np.mean(gap >= -1e-6)))
+                frac_learned_not_worse=float(np.mean(gap >= -1e-6)),
+                frac_learned_kept=float(np.mean([r.get("learned_kept", True) for r in rec])))
 
 
 def main(argv=None):

@@ -104,3 +104,68 @@ def test_ordering_batch_bit_exact(gpu):
         ref = O.maximin_order(X)
         np.testing.assert_array_equal(order, ref)
         np.testing.assert_array_equal(nn, O.find_ordered_nn(X[ref], 12))
+
+
+def _batch_run(bh, theta0, var_penalty, use_graph):
+    """Fisher scoring to the end on a fixed structure (reorders declined)."""
+    bh.fit_start(theta0, 40, 1e-4, var_penalty, use_graph)
+    for _ in range(10000):
+        st = bh.fit_step(1)
+        for i in np.flatnonzero(st == 2):
+            bh.set_reorder(int(i), False)
+        if np.all((st == 3) | (st == 4)):
+            break
+    return bh.get()
+
+
+def test_graph_recaptured_when_var_penalty_changes(gpu):
+    """fit_start twice on one set-up with different var_penalty: the replayed
+    graph must use the new value (same result as the uncaptured steps)."""
+    from pyscaledvecchia_amd import _lib
+    from pyscaledvecchia_amd.batch import BatchedFit
+    from pyscaledvecchia_amd.gp import ScaledVecchiaGP
+    data = _datasets(21, nd=2)
+    models = [ScaledVecchiaGP(m_est=15, nu=2.5, trend="constant", n_est=2000) for _ in data]
+    bf = BatchedFit(models)
+    preps = [g._prepare(X, y) for g, (X, y) in zip(models, data)]
+    structs = [bf._structure(g, p, p["theta"], 15) for g, p in zip(models, preps)]
+    bh = _lib.BatchHandle(gpu)
+    bh.setup([p["d"] for p in preps], [1, 1], [2, 2], [len(s["yo"]) for s in structs],
+             np.concatenate([s["Xo"].ravel() for s in structs]),
+             np.concatenate([s["yo"] for s in structs]),
+             np.concatenate([s["Zo"].ravel() for s in structs]),
+             np.concatenate([s["nbr"] for s in structs]),
+             np.concatenate([s["kcnt"] for s in structs]),
+             np.concatenate([s["chunk"] for s in structs]), [s["nchunk"] for s in structs], 15,
+             [p["lvt"] - 1.0 for p in preps], [0, 0], [0.0, 0.0], 1e3)
+    th0 = np.zeros((2, 12))
+    for i, p in enumerate(preps):
+        th0[i, :p["d"] + 2] = p["theta"]
+    a = _batch_run(bh, th0, 1.0, True)       # captures the graph with var_penalty 1
+    b = _batch_run(bh, th0, 10.0, True)      # must capture again
+    c = _batch_run(bh, th0, 10.0, False)     # plain launches, var_penalty 10
+    np.testing.assert_array_equal(b[0], c[0])
+    np.testing.assert_array_equal(b[1], c[1])
+    assert not np.array_equal(a[1], b[1])
+
+
+def test_batch_setup_rejects_null_trend_with_columns(gpu):
+    """The C ABI refuses Z = NULL with data given and q > 0 (no host read of a
+    NULL pointer)."""
+    import ctypes
+    L = gpu._L
+    i32 = lambda v: np.ascontiguousarray(v, np.int32)    # noqa: E731
+    d, q, nu, nc = i32([2]), i32([1]), i32([2]), i32([10])
+    n = np.array([10], np.int64)
+    X, y = np.random.rand(20), np.random.rand(10)
+    chunk = i32(np.arange(10))
+    lvt, lnug = np.zeros(1), np.zeros(1)
+    fixed = i32([0])
+    P = ctypes.POINTER
+    dp = lambda a: a.ctypes.data_as(P(ctypes.c_double))   # noqa: E731
+    ip = lambda a: a.ctypes.data_as(P(ctypes.c_int32))    # noqa: E731
+    rc = L.svh_batch_setup(gpu._h, 1, ip(d), ip(q), ip(nu), n.ctypes.data_as(P(ctypes.c_int64)),
+                           dp(X), dp(y), None, None, None, ip(chunk), ip(nc), 5, dp(lvt),
+                           ip(fixed), dp(lnug), 1e3)
+    assert rc == 2
+    assert b"Z is NULL" in L.svh_last_error(gpu._h)

@@ -244,3 +244,22 @@ def test_predict_other_smoothness_against_oracle(gpu, nu):
         mu, va = gpu.predict(X, resid, Xt, m, theta, nu)
         assert _rel(mu, mu0) < 1e-9
         assert _rel(va, va0) < 1e-8
+
+
+@pytest.mark.parametrize("log_var", [math.log(1e6), math.log(1e-6)])
+def test_predict_extreme_scales_and_tiny_nugget(gpu, log_var):
+    """Prediction blocks with nugget 1e-12 and variance scales 1e6 / 1e-6: the
+    pivots of the matrix-core kernel's diagonal tiles (hardware reciprocal
+    square root refined by Newton steps) against the oracle."""
+    rng = np.random.default_rng(91)
+    n, d, ns, m = 2500, 3, 120, 140
+    X = rng.random((n, d))
+    resid = np.sin(4 * X[:, 0]) + X[:, 1] * X[:, 2]
+    theta = np.array([log_var, math.log(0.5), math.log(0.7), math.log(1.1), math.log(1e-12)])
+    Xt = rng.random((ns, d))
+    og = O.OracleGP(m_pred=m, nu=2.5)
+    og.theta_, og.d_ = theta, d
+    mu0, va0 = og._predict_scaled(Xt, X, resid, m)
+    mu, va = gpu.predict(X, resid, Xt, m, theta, 2.5)
+    assert _rel(mu, mu0) < 1e-9
+    assert _rel(va, va0) < 1e-8

@@ -1,5 +1,7 @@
 """Learned warm start (the packaged network) on the GPU: what it gains over
-the default start (gp.py:169-177), and the fallback to that start."""
+the default start (gp.py:169-177), and the guarantee that a learned start
+never ends below the fit from that start (both fits run, the higher final
+log-likelihood is kept)."""
 
 import numpy as np
 import pytest
@@ -50,7 +52,7 @@ def test_warm_start_single_fit(gpu):
 
 
 class _BadStart:
-    """A 'network' proposing absurd ranges: the start guard must reject it."""
+    """A 'network' proposing absurd ranges: its fit must lose to the default."""
 
     def theta0(self, Xe, ye, nu):
         d = Xe.shape[1]
@@ -69,11 +71,28 @@ def test_rejected_learned_start_reproduces_default_fit(gpu):
     ref = ScaledVecchiaGP(m_est=15, var_correction=False).fit(X, y)
     g = ScaledVecchiaGP(m_est=15, var_correction=False, warm_start=_BadStart()).fit(X, y)
     assert g.warm_start_used_ is False
-    assert g.loglik_ == ref.loglik_
-    np.testing.assert_array_equal(g.theta_, ref.theta_)
+    # the kept fit is the batched fitter's default-start fit
+    assert g.loglik_ == pytest.approx(ref.loglik_, rel=1e-9)
+    np.testing.assert_allclose(g.theta_, ref.theta_, rtol=1e-7, atol=1e-7)
     # batched path: same guard
     b_ref = ScaledVecchiaGP(m_est=15, var_correction=False)
     b_bad = ScaledVecchiaGP(m_est=15, var_correction=False, warm_start=_BadStart())
     BatchedFit([b_ref, b_bad]).fit([X, X], [y, y])
     assert b_bad.warm_start_used_ is False
     assert b_bad.loglik_ == b_ref.loglik_
+
+
+def test_learned_start_never_ends_below_default_c2(gpu):
+    """C2 data (n = 50,000, d = 8, linear trend, m_est = 30): the learned-start
+    fit's final log-likelihood is never below the default-start fit's."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import c2_data
+    from pyscaledvecchia_amd import ScaledVecchiaGP
+    X, y = c2_data()
+    kw = dict(m_est=30, m_pred=140, nu=2.5, trend="linear", n_est=50_000, var_correction=False)
+    ref = ScaledVecchiaGP(**kw).fit(X, y)
+    g = ScaledVecchiaGP(warm_start="learned", **kw).fit(X, y)
+    print(f"\nC2: default {ref.loglik_:.6f} learned-start fit {g.loglik_:.6f} "
+          f"(learned kept: {g.warm_start_used_})")
+    assert g.loglik_ >= ref.loglik_ - 1e-9 * abs(ref.loglik_)
