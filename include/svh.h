@@ -49,6 +49,14 @@ void svh_destroy(svh_ctx* ctx);
 /* Message of the last failed call on this context ("" if none). */
 const char* svh_last_error(const svh_ctx* ctx);
 
+/* General Matern correlation on the host: M[i] = M_nu(r[i]) and g[i] =
+ * M_nu'(r[i]) / r[i] for any nu > 0 (2^(1-nu)/Gamma(nu) c^nu K_nu(c),
+ * c = sqrt(2 nu) r; M = 1 and g = 0 below r = 1e-7), by the same code the
+ * kernels run for smoothness outside {0.5, 1.5, 2.5}.  Needs no device or
+ * context.  Replaces _matern_corr_general / _matern_g_general
+ * (_matern_general.py:92-116). */
+int svh_matern_general(double nu, const double* r, int64_t n, double* M, double* g);
+
 /* ---------------------------------------------------------------- (a) --- */
 
 /* Exact maximin order of the n rows of X (n x d, scaled coordinates),
@@ -104,11 +112,17 @@ int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const do
                       int32_t nchunk_val);
 
 /* One profile Vecchia log-likelihood evaluation on the uploaded structure.
- *   theta (P = d+2): log sigma^2, log lambda_1..d, log tau.  nu in {0.5,1.5,2.5}.
+ *   theta (P = d+2): log sigma^2, log lambda_1..d, log tau, for a fixed
+ *   smoothness nu > 0 (closed forms at 0.5, 1.5, 2.5; any other value uses
+ *   the Bessel form of _matern_general.py:92-116).  nu == 0: the smoothness
+ *   is estimated, theta (P = d+3) = log sigma^2, log lambda_1..d, log nu,
+ *   log tau, and the log-nu derivative is a central difference in nu with
+ *   step 1e-3 (_matern_general.py:134-179); the structure's gradient jitter
+ *   chunks must then follow P = d+3 (svh_vecchia_rebuild provides both).
  *   Outputs: ll, grad[P], fisher[P*P] (row-major), beta[q]; grad/fisher are
  *   untouched when need_grad == 0.  *jitter_power_max receives the largest
  *   jitter power used by any chunk (0 = none, else -12..-4).
- * Replaces likelihood.py:66-175 (vecchia_profile_loglik) for fixed nu. */
+ * Replaces likelihood.py:66-175 (vecchia_profile_loglik), fixed or estimated nu. */
 /* The whole rebuild of a fit at new ranges, on the device: the estimation
  * data Xe (n x d), ye (n), Ze (n x q, NULL when q == 0) -- uploaded when
  * upload_data is nonzero, otherwise the copy of the previous call with the
@@ -139,7 +153,8 @@ int svh_vecchia_eval(svh_ctx* ctx, const double* theta, int32_t P, double nu,
  * (noise_free != 0), and
  *   mean[i] = -a[0:m].resid[nb] / a[m],   var[i] = 1 / a[m]^2
  * where a = L^-T e_K.  Xtr (ntr x d) and Xq are standardised inputs, resid
- * (ntr) the centred responses, theta as in svh_vecchia_eval.
+ * (ntr) the centred responses, theta = log sigma^2, log lambda_1..d, log tau
+ * (d+2 entries) and nu > 0 the smoothness (a fitted one too).
  * nb_out (nq x m) receives the neighbour indices when not NULL. */
 int svh_predict(svh_ctx* ctx, const double* Xtr, const double* resid, int64_t ntr,
                 const double* Xq, int64_t nq, int32_t d, int32_t m, const double* theta,

@@ -23,6 +23,7 @@ import math
 
 import numpy as np
 import scipy.linalg
+import scipy.special
 from scipy import sparse
 from scipy.sparse.linalg import spsolve_triangular
 
@@ -62,7 +63,7 @@ def matern_corr(r, nu):
     if nu == 2.5:
         c = SQRT5 * r
         return (1.0 + c + c * c / 3.0) * np.exp(-c)
-    raise ValueError("nu must be one of 0.5, 1.5, 2.5")
+    return matern_corr_general(r, nu)     # _covariance.py:178-183 falls back
 
 
 def matern_g(r, nu):
@@ -76,7 +77,32 @@ def matern_g(r, nu):
     if nu == 2.5:
         c = SQRT5 * r
         return -(5.0 / 3.0) * (1.0 + c) * np.exp(-c)
-    raise ValueError("nu must be one of 0.5, 1.5, 2.5")
+    return matern_g_general(r, nu)
+
+
+R_FLOOR = 1e-7     # _matern_general.py:88
+
+
+def matern_corr_general(r, nu):
+    """M_nu(r) = 2^(1-nu)/Gamma(nu) c^nu K_nu(c), c = sqrt(2 nu) r; 1 below
+    the distance floor (_matern_general.py:92-99)."""
+    r = np.asarray(r, dtype=np.float64)
+    rr = np.where(r < R_FLOOR, R_FLOOR, r)
+    c = math.sqrt(2.0 * nu) * rr
+    lc = (1.0 - nu) * math.log(2.0) - scipy.special.gammaln(nu)
+    val = np.exp(lc + nu * np.log(c)) * scipy.special.kv(nu, c)
+    return np.where(r < R_FLOOR, 1.0, val)
+
+
+def matern_g_general(r, nu):
+    """g(r) = M_nu'(r)/r via d/dz[z^nu K_nu(z)] = -z^nu K_{nu-1}(z); 0 below
+    the floor (_matern_general.py:102-116)."""
+    r = np.asarray(r, dtype=np.float64)
+    rr = np.where(r < R_FLOOR, R_FLOOR, r)
+    c = math.sqrt(2.0 * nu) * rr
+    lc = (1.0 - nu) * math.log(2.0) - scipy.special.gammaln(nu)
+    val = -(math.sqrt(2.0 * nu) / rr) * np.exp(lc + nu * np.log(c)) * scipy.special.kv(nu - 1.0, c)
+    return np.where(r < R_FLOOR, 0.0, val)
 
 
 # --------------------------------------------------------------------------
@@ -165,8 +191,11 @@ def knn_query(Xref, Xq, k):
 # block covariance, Cholesky, triangular solves (_covariance.py:86-259)
 # --------------------------------------------------------------------------
 
-def block_cov(Xb, ranges, variance, nugget, nu, derivs=True, nug_mask=None):
-    """Sigma (B,K,K) and dSigma/dlog-theta (B,P,K,K), P = d+2.
+def block_cov(Xb, ranges, variance, nugget, nu, derivs=True, nug_mask=None,
+              estimate_nu=False, nu_fd_h=1e-3):
+    """Sigma (B,K,K) and dSigma/dlog-theta (B,P,K,K), P = d+2 (d+3 with
+    estimate_nu: the log nu column, nu * central difference of M in nu with
+    step 1e-3, sits before the nugget's, _matern_general.py:134-179).
 
     _covariance.py:86-147 (kernels) and 150-198 (dispatch).  Sigma_ij =
     variance*(M(r_ij) + nugget*mask_i*[i==j]); dS_0 = Sigma; dS_{1+l} =
@@ -187,11 +216,16 @@ def block_cov(Xb, ranges, variance, nugget, nu, derivs=True, nug_mask=None):
     if not derivs:
         return Sigma, None
     g = matern_g(r, nu)
-    dS = np.empty((B, d + 2, K, K))
+    P = d + 2 + (1 if estimate_nu else 0)
+    dS = np.empty((B, P, K, K))
     dS[:, 0] = Sigma
     for l in range(d):
         dS[:, 1 + l] = -variance * g * diff[..., l] * diff[..., l]
-    dS[:, 1 + d] = variance * nug
+    if estimate_nu:
+        dMnu = (matern_corr_general(r, nu + nu_fd_h)
+                - matern_corr_general(r, nu - nu_fd_h)) / (2.0 * nu_fd_h)
+        dS[:, 1 + d] = variance * nu * dMnu
+    dS[:, P - 1] = variance * nug
     return Sigma, dS
 
 
@@ -258,7 +292,8 @@ def chunks(n_rows, K, P, budget=CHUNK_BUDGET):
 # --------------------------------------------------------------------------
 
 def vecchia_profile_loglik(theta, Xo, yo, Zo, groups, nu, need_grad=True,
-                           var_penalty=0.0, log_var_target=0.0, report=None):
+                           var_penalty=0.0, log_var_target=0.0, report=None,
+                           estimate_nu=False):
     """Two-pass evaluation of likelihood.py:66-175 (fixed nu only).
 
     Pass 1 per chunk: Sigma/dS, Cholesky, a = L^-T e_K, GLS normal equations,
@@ -270,8 +305,13 @@ def vecchia_profile_loglik(theta, Xo, yo, Zo, groups, nu, need_grad=True,
     d = Xo.shape[1]
     variance = math.exp(theta[0])
     ranges = np.exp(theta[1:1 + d])
-    nugget = math.exp(theta[1 + d])
-    P = d + 2
+    if estimate_nu:                           # likelihood.py:90-93
+        nu = math.exp(theta[1 + d])
+        nugget = math.exp(theta[2 + d])
+        P = d + 3
+    else:
+        nugget = math.exp(theta[1 + d])
+        P = d + 2
     n = Xo.shape[0]
     has_Z = Zo is not None and Zo.shape[1] > 0
     q = Zo.shape[1] if has_Z else 0
@@ -285,7 +325,8 @@ def vecchia_profile_loglik(theta, Xo, yo, Zo, groups, nu, need_grad=True,
         idx = np.concatenate([nb, rows[:, None]], axis=1)
         for a0, a1 in chunks(len(rows), K, P if need_grad else 1):
             block = idx[a0:a1]
-            Sig, dS = block_cov(Xo[block], ranges, variance, nugget, nu, derivs=need_grad)
+            Sig, dS = block_cov(Xo[block], ranges, variance, nugget, nu, derivs=need_grad,
+                                estimate_nu=estimate_nu)
             L, p = batch_chol(Sig)
             jit.append(p)
             E = np.zeros((a1 - a0, K, 1))
@@ -402,10 +443,12 @@ def design(X, trend):
     raise ValueError("trend must be 'zero', 'constant' or 'linear'")
 
 
-def clip_theta(theta, d, lambda_max=1e3):
-    """gp.py:126-135 (fixed nu)."""
+def clip_theta(theta, d, lambda_max=1e3, estimate_nu=False):
+    """gp.py:126-135."""
     hi = math.log(lambda_max) if np.isfinite(lambda_max) else np.inf
     theta[1:1 + d] = np.clip(theta[1:1 + d], math.log(1e-8), hi)
+    if estimate_nu:
+        theta[1 + d] = np.clip(theta[1 + d], math.log(0.3), math.log(10.0))
     theta[-1] = np.clip(theta[-1], math.log(1e-12), math.log(1e3))
     return theta
 
@@ -418,6 +461,7 @@ class OracleGP:
                  nugget=None, var_penalty=1.0, var_correction=True,
                  lambda_max=1e3, max_iter=40, tol=1e-4, random_state=0):
         self.m_est, self.m_pred, self.n_est, self.nu = m_est, m_pred, n_est, nu
+        self._nu_fit = nu
         self.trend, self.nugget, self.var_penalty = trend, nugget, var_penalty
         self.var_correction, self.lambda_max = var_correction, lambda_max
         self.max_iter, self.tol, self.random_state = max_iter, tol, random_state
@@ -428,6 +472,10 @@ class OracleGP:
     @property
     def ranges_(self):
         return np.exp(self.theta_[1:1 + self.d_])
+
+    @property
+    def nu_(self):
+        return self._nu_fit
 
     def fit(self, X, y):
         """gp.py:138-239."""
@@ -451,12 +499,16 @@ class OracleGP:
         Xe, ye = self.X_[sub], self.y_[sub]
         Ze = design(Xe, self.trend)
         m = min(self.m_est, len(sub) - 1)
-        theta = np.empty(d + 2)
+        est = self.nu is None                     # gp.py:145, 169-177
+        self._estimate_nu = est
+        theta = np.empty(d + 2 + (1 if est else 0))
         theta[0] = math.log(max(ye.var(), 1e-8))
         theta[1:1 + d] = math.log(0.2 * math.sqrt(d))
+        if est:
+            theta[1 + d] = math.log(1.5)
         fixed = self.nugget is not None
         theta[-1] = math.log(self.nugget if fixed else 0.1)
-        theta = clip_theta(theta, d, self.lambda_max)
+        theta = clip_theta(theta, d, self.lambda_max, est)
         self.theta0_ = theta.copy()
         state = {}
 
@@ -473,20 +525,21 @@ class OracleGP:
         lvt = math.log(max(ye.var(), 1e-8))
 
         def objective(th):
-            th = clip_theta(np.array(th, dtype=float), d, self.lambda_max)
+            th = clip_theta(np.array(th, dtype=float), d, self.lambda_max, est)
             if fixed:
                 th[-1] = math.log(self.nugget)
             self.n_evals += 1
             ll, g, M, beta = vecchia_profile_loglik(
                 th, state["Xo"], state["yo"], state["Zo"], state["groups"], self.nu,
-                need_grad=True, var_penalty=self.var_penalty, log_var_target=lvt)
+                need_grad=True, var_penalty=self.var_penalty, log_var_target=lvt,
+                estimate_nu=est)
             if fixed:
                 g = g.copy(); g[-1] = 0.0
                 M = M.copy(); M[-1, :] = 0.0; M[:, -1] = 0.0; M[-1, -1] = 1.0
             return ll, g, M, beta
 
         def reorder_hook(th, k):
-            r = np.exp(clip_theta(np.array(th), d, self.lambda_max)[1:1 + d])
+            r = np.exp(clip_theta(np.array(th), d, self.lambda_max, est)[1:1 + d])
             if np.max(np.abs(np.log(r / last[0]))) < 1e-3:
                 return False
             rebuild(th)
@@ -496,10 +549,11 @@ class OracleGP:
         theta, ll, beta = fisher_scoring(theta, objective, max_iter=self.max_iter,
                                          tol=self.tol, reorder_hook=reorder_hook,
                                          trace=self.trace)
-        theta = clip_theta(np.array(theta), d, self.lambda_max)
+        theta = clip_theta(np.array(theta), d, self.lambda_max, est)
         if fixed:
             theta[-1] = math.log(self.nugget)
         self.theta_, self.beta_, self.loglik_ = theta, beta, ll
+        self._nu_fit = math.exp(theta[1 + d]) if est else self.nu     # gp.py:233
         self.state_ = state
         self.b_ = 1.0
         if self.var_correction and n >= 200:
@@ -523,7 +577,7 @@ class OracleGP:
             mask = np.ones((a1 - a0, K))
             if noise_free:
                 mask[:, K - 1] = 0.0
-            Sig, _ = block_cov(Xb, ranges, var, nug, self.nu, derivs=False, nug_mask=mask)
+            Sig, _ = block_cov(Xb, ranges, var, nug, self.nu_, derivs=False, nug_mask=mask)
             L, _ = batch_chol(Sig)
             E = np.zeros((a1 - a0, K, 1))
             E[:, K - 1, 0] = 1.0
@@ -574,7 +628,7 @@ class OracleGP:
             mask = np.ones((a1 - a0, K))
             mask[:, K - 1] = 0.0
             mask[:, :K - 1] = (block[:, :K - 1] < n).astype(float)
-            Sig, _ = block_cov(Xo[block], ranges, var, nug, self.nu, derivs=False, nug_mask=mask)
+            Sig, _ = block_cov(Xo[block], ranges, var, nug, self.nu_, derivs=False, nug_mask=mask)
             L, _ = batch_chol(Sig)
             E = np.zeros((a1 - a0, K, 1))
             E[:, K - 1, 0] = 1.0

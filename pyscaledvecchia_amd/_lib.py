@@ -66,6 +66,8 @@ def _load():
     L.svh_predict_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64,
                                       ctypes.c_int32, ctypes.c_int32, _dp, ctypes.c_double,
                                       ctypes.c_int32, ctypes.c_int32, _vp, _vp]
+    L.svh_matern_general.argtypes = [ctypes.c_double, _dp, ctypes.c_int64, _dp, _dp]
+    L.svh_matern_general.restype = ctypes.c_int
     L.svh_block_columns.argtypes = [_vp, _dp, ctypes.c_int64, ctypes.c_int32, _i64p, _dp,
                                     ctypes.c_int64, ctypes.c_int32, _dp, ctypes.c_double,
                                     ctypes.c_int32, _dp]
@@ -127,7 +129,19 @@ EXPORTED = ("svh_device_count", "svh_version", "svh_create", "svh_destroy", "svh
             "svh_mlp_set_params", "svh_mlp_get_params", "svh_mlp_forward", "svh_mlp_loss_grad",
             "svh_mlp_load_data", "svh_mlp_train_epoch", "svh_simulate_rff", "svh_autotune",
             "svh_autotune_read", "svh_vecchia_rebuild", "svh_batch_data", "svh_batch_rebuild",
-            "svh_joint_solve", "svh_ordered_knn_rows", "svh_predict_noalloc")
+            "svh_joint_solve", "svh_ordered_knn_rows", "svh_predict_noalloc",
+            "svh_matern_general")
+
+
+def matern_general(nu, r):
+    """Host evaluation of the general Matern M_nu(r) and g = M'/r (the
+    kernels' code for smoothness outside {0.5, 1.5, 2.5})."""
+    r = np.ascontiguousarray(r, dtype=np.float64).ravel()
+    M, g = np.empty_like(r), np.empty_like(r)
+    rc = _load().svh_matern_general(float(nu), _ptr(r), r.size, _ptr(M), _ptr(g))
+    if rc != SVH_OK:
+        raise ValueError("matern_general: nu must be positive")
+    return M, g
 
 
 def device_count() -> int:

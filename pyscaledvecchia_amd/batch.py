@@ -48,6 +48,8 @@ class BatchedFit:
         self.device = device
         m0 = self.models[0]
         for g in self.models:
+            if g.nu not in _NU_CODE:
+                raise ValueError("batched fits need nu in {0.5, 1.5, 2.5}")
             if (g.var_penalty, g.max_iter, g.tol, g.lambda_max) != \
                     (m0.var_penalty, m0.max_iter, m0.tol, m0.lambda_max):
                 raise ValueError("batched models must share var_penalty, max_iter, tol, "

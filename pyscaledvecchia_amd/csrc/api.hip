@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "svh_internal.h"
+#include "bessel.h"
 
 #include <cmath>
 
@@ -506,3 +507,10 @@ int svh_timing_reset(svh_ctx* ctx) {
 }
 
 }  // extern "C"
+
+int svh_matern_general(double nu, const double* r, int64_t n, double* M, double* g) {
+  if (!(nu > 0.0) || n < 0 || (n > 0 && (!r || !M || !g))) return SVH_ERR_ARG;
+  const svh_bessel::MaternNu m = svh_bessel::matern_nu(nu);
+  for (int64_t i = 0; i < n; ++i) svh_bessel::matern_general(m, r[i], M[i], g[i]);
+  return SVH_OK;
+}

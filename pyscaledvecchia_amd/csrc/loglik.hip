@@ -35,6 +35,7 @@
 // z_K = u - v.beta and W_p.z = s_p - T_p.beta exactly (linearity of L^-1).
 
 #include "svh_internal.h"
+#include "bessel.h"
 
 #include <cmath>
 #include <cstddef>
@@ -92,7 +93,14 @@ struct LLArgs {
   unsigned long long* prof;  // per-phase shader-clock sums (profiling runs), or NULL
   int zmode;       // trend columns: 0 gathered from Z, 1 Z = [1, x], 2 Z = 1
   int sig32;       // mixed kernels: off-diagonal covariance entries in float32
+  // nu_code 3: general smoothness (Bessel K_nu); with est_nu the log-nu
+  // derivative column (central difference in nu, steps gnu_p / gnu_m) sits
+  // at 1 + d, before the nugget's
+  svh_bessel::MaternNu gnu, gnu_p, gnu_m;
+  int est_nu;
 };
+
+constexpr double kNuStep = 1e-3;   // central-difference step in nu (_matern_general.py:135)
 
 __device__ __forceinline__ void matern(int nu_code, double r, double& M, double& g) {
   if (nu_code == 0) {  // nu = 0.5
@@ -132,7 +140,8 @@ __global__ __launch_bounds__(kWave) void ll_pass1(LLArgs a) {
   double* R = S + KP * LD;          // NR * KP
   double* av = R + NR * KP;         // KP
   double* acc = av + KP;            // NACC
-  int* pts = reinterpret_cast<int*>(acc + NACC);  // KP
+  double* Dn = acc + NACC;          // KP * LD when the smoothness is estimated
+  int* pts = reinterpret_cast<int*>(Dn + (a.est_nu ? KP * LD : 0));  // KP
 
   for (int e = lane; e < NACC; e += kWave) acc[e] = 0.0;
 
@@ -165,12 +174,23 @@ __global__ __launch_bounds__(kWave) void ll_pass1(LLArgs a) {
         s += t * t;
       }
       double M, g;
-      matern(a.nu_code, sqrt(s), M, g);
+      const double rr = sqrt(s);
+      if (a.nu_code == 3)
+        svh_bessel::matern_general(a.gnu, rr, M, g);
+      else
+        matern(a.nu_code, rr, M, g);
       if (i == j) {
         S[i * LD + i] = a.var * (M + a.nug);
+        if (a.est_nu) Dn[i * LD + i] = 0.0;
       } else {
         S[i * LD + j] = a.var * M;
         S[j * LD + i] = g;
+        if (a.est_nu) {
+          const double dm = (svh_bessel::matern_general_M(a.gnu_p, rr) -
+                             svh_bessel::matern_general_M(a.gnu_m, rr)) / (2.0 * kNuStep);
+          Dn[i * LD + j] = dm;
+          Dn[j * LD + i] = dm;
+        }
       }
     }
     __syncthreads();
@@ -239,6 +259,14 @@ __global__ __launch_bounds__(kWave) void ll_pass1(LLArgs a) {
           s += g * (t * t) * av[j];
         }
         R[(1 + l) * KP + i] = -a.var * s;
+      }
+      if (a.est_nu) {
+        // d Sigma / d log nu = var nu dM/dnu (zero diagonal)
+        for (int i = lane; i < K; i += kWave) {
+          double s = 0.0;
+          for (int j = 0; j < K; ++j) s += Dn[i * LD + j] * av[j];
+          R[(1 + d) * KP + i] = a.var * a.gnu.nu * s;
+        }
       }
       __syncthreads();
     }
@@ -1583,6 +1611,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
   a.prof = nullptr;
   a.zmode = 0;
   a.sig32 = 0;
+  a.est_nu = 0;
   for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
     split_block<-1, false, false>(a, b, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
@@ -1632,6 +1661,7 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
   a.prof = nullptr;
   a.zmode = 0;
   a.sig32 = 0;
+  a.est_nu = 0;
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2070,6 +2100,7 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
   up("v.nbr", hnbr.data(), hnbr.size() * sizeof(int));
   up("v.kcnt", kcnt, (size_t)nblk * sizeof(int));
   up("v.cg", chunk_grad, (size_t)nblk * sizeof(int));
+  up("v.cgn", chunk_grad, (size_t)nblk * sizeof(int));
   up("v.cv", chunk_val, (size_t)nblk * sizeof(int));
   SVH_HIP(hipStreamSynchronize(s));
   vs.n = n;
@@ -2079,6 +2110,7 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
   vs.mmax = mmax;
   vs.kp = (mmax + 1 <= 32) ? 32 : (mmax + 1 <= 64 ? 64 : 128);
   vs.nchunk_grad = nchunk_grad;
+  vs.nchunk_grad_nu = nchunk_grad;   // the caller's gradient chunks serve both layouts
   vs.nchunk_val = nchunk_val;
   vs.zmode = q > 0 ? trend_mode(Xo, Zo, n, d, q) : 0;
   vs.ready = true;
@@ -2093,10 +2125,11 @@ void vecchia_setup_device(svh_ctx* ctx, const double* Xo, const double* yo, cons
 __global__ void rebuild_rows(const double* __restrict__ Xe, const double* __restrict__ ye,
                              const double* __restrict__ Ze, const int64_t* __restrict__ order,
                              const int64_t* __restrict__ nn, int64_t n, int d, int q, int m,
-                             int mcol, int64_t stepg, int64_t stepv, double* __restrict__ X,
-                             double* __restrict__ y, double* __restrict__ Z, int* __restrict__ rows,
+                             int mcol, int64_t stepg, int64_t stepv, int64_t stepn,
+                             double* __restrict__ X, double* __restrict__ y,
+                             double* __restrict__ Z, int* __restrict__ rows,
                              int* __restrict__ nbr, int* __restrict__ kcnt, int* __restrict__ cg,
-                             int* __restrict__ cv) {
+                             int* __restrict__ cv, int* __restrict__ cn) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t src = order[i];
@@ -2109,6 +2142,7 @@ __global__ void rebuild_rows(const double* __restrict__ Xe, const double* __rest
   for (int t = 0; t < mcol; ++t) nbr[i * mcol + t] = t < k ? (int)nn[i * (m + 1) + 1 + t] : -1;
   cg[i] = (int)(i < m ? i : m + (i - m) / stepg);
   cv[i] = (int)(i < m ? i : m + (i - m) / stepv);
+  cn[i] = (int)(i < m ? i : m + (i - m) / stepn);
 }
 
 void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, const double* Ze,
@@ -2152,6 +2186,9 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   const int P = d + 2, K = m + 1, mcol = m > 0 ? m : 1;
   const int64_t stepg = std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K * P));
   const int64_t stepv = std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K));
+  // gradient chunks with the smoothness estimated (P = d + 3)
+  const int64_t stepn =
+      std::max<int64_t>(1, 4000000 / std::max<int64_t>(1, (int64_t)K * K * (P + 1)));
   double* vX = ctx->arena.get_t<double>("v.X", (size_t)n * d);
   double* vy = ctx->arena.get_t<double>("v.y", n);
   double* vZ = ctx->arena.get_t<double>("v.Z", (size_t)n * (q > 0 ? q : 1));
@@ -2160,9 +2197,10 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   int* vk = ctx->arena.get_t<int>("v.kcnt", n);
   int* vcg = ctx->arena.get_t<int>("v.cg", n);
   int* vcv = ctx->arena.get_t<int>("v.cv", n);
+  int* vcn = ctx->arena.get_t<int>("v.cgn", n);
   hipLaunchKernelGGL(rebuild_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dXe, dye,
-                     dZe, dord, dnn, n, d, q, m, mcol, stepg, stepv, vX, vy, vZ, vrows, vnbr, vk,
-                     vcg, vcv);
+                     dZe, dord, dnn, n, d, q, m, mcol, stepg, stepv, stepn, vX, vy, vZ, vrows,
+                     vnbr, vk, vcg, vcv, vcn);
   SVH_HIP(hipGetLastError());
   SVH_HIP(hipMemcpyAsync(order, dord, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SVH_HIP(hipStreamSynchronize(s));
@@ -2176,6 +2214,7 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   vs.kp = (m + 1 <= 32) ? 32 : (m + 1 <= 64 ? 64 : 128);
   vs.nchunk_grad = (int)(m + (rest + stepg - 1) / stepg);
   vs.nchunk_val = (int)(m + (rest + stepv - 1) / stepv);
+  vs.nchunk_grad_nu = (int)(m + (rest + stepn - 1) / stepn);
   vs.zmode = ctx->rb_zmode;
   vs.ready = true;
 }
@@ -2185,18 +2224,27 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
                          double* grad, double* fisher, double* beta, int* jitter_power_max) {
   VecchiaState& vs = ctx->vs;
   SVH_REQUIRE(vs.ready, "eval: call svh_vecchia_setup first");
-  SVH_REQUIRE(P == vs.d + 2, "eval: theta must have d+2 entries");
+  const bool est_nu = nu == 0.0;   // smoothness estimated: theta[1 + d] = log nu
+  SVH_REQUIRE(P == vs.d + (est_nu ? 3 : 2),
+              est_nu ? "eval: theta must have d+3 entries when nu is estimated"
+                     : "eval: theta must have d+2 entries");
   SVH_REQUIRE(precision == SVH_FP64 || precision == SVH_MIXED || precision == SVH_MIXED_F32COV,
               "eval: bad precision mode");
   int nu_code;
-  if (nu == 0.5)
+  const double nuv = est_nu ? std::exp(theta[1 + vs.d]) : nu;
+  if (est_nu)
+    nu_code = 3;
+  else if (nu == 0.5)
     nu_code = 0;
   else if (nu == 1.5)
     nu_code = 1;
   else if (nu == 2.5)
     nu_code = 2;
+  else if (nu > 0.0 && std::isfinite(nu))
+    nu_code = 3;
   else
-    throw SvhError{SVH_ERR_ARG, "eval: nu must be 0.5, 1.5 or 2.5"};
+    throw SvhError{SVH_ERR_ARG, "eval: nu must be positive (or 0: estimated)"};
+  SVH_REQUIRE(std::isfinite(nuv) && nuv > kNuStep, "eval: smoothness out of range");
   const int d = vs.d, q = vs.q;
   hipStream_t s = ctx->stream;
   // parameters
@@ -2205,7 +2253,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   double* inv_lam = ctx->arena.get_t<double>("v.invlam", 32);
   SVH_HIP(hipMemcpyAsync(inv_lam, hpar, d * sizeof(double), hipMemcpyHostToDevice, s));
 
-  const int nchunk = need_grad ? vs.nchunk_grad : vs.nchunk_val;
+  const int nchunk = need_grad ? (est_nu ? vs.nchunk_grad_nu : vs.nchunk_grad) : vs.nchunk_val;
   int* level = ctx->arena.get_t<int>("v.level", nchunk);
   int* fail = ctx->arena.get_t<int>("v.fail", nchunk);
   int_fill(ctx, level, 0, nchunk);
@@ -2220,7 +2268,8 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   // SVH_LL_KERNEL = rows | split | lds forces a kernel variant (A/B timing)
   const char* kenv = std::getenv("SVH_LL_KERNEL");
   const std::string kv = kenv ? kenv : "";
-  const bool fits32 = vs.mmax + 1 <= kSK && d <= kSD && q <= kSQ;
+  // the general-smoothness path (Bessel K_nu) runs in the LDS kernel only
+  const bool fits32 = vs.mmax + 1 <= kSK && d <= kSD && q <= kSQ && nu_code < 3;
   const bool use_split = fits32 && kv != "lds";
   const bool use_rows = fits32 && kv != "lds" && kv != "split";
   const size_t lds_split = (size_t)split_lds_doubles(d) * sizeof(double);
@@ -2262,7 +2311,8 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     while (G2L < 1 + P) G2L <<= 1;
   SVH_REQUIRE(G2L <= 256, "pass 2: too many parameters");
   const int G2 = (int)std::min<int64_t>((vs.nblk * G2L + 255) / 256, 2048);
-  const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC) * sizeof(double) +
+  const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC +
+                             (est_nu ? KP * (KP + 1) : 0)) * sizeof(double) +
                      (size_t)KP * sizeof(int);
   SVH_REQUIRE(lds <= 160 * 1024, "eval: block too large for LDS");
 
@@ -2273,7 +2323,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.rows = (const int*)ctx->arena.get("v.rows", 0);
   a.nbr = (const int*)ctx->arena.get("v.nbr", 0);
   a.kcnt = (const int*)ctx->arena.get("v.kcnt", 0);
-  a.chunk = (const int*)ctx->arena.get(need_grad ? "v.cg" : "v.cv", 0);
+  a.chunk = (const int*)ctx->arena.get(need_grad ? (est_nu ? "v.cgn" : "v.cg") : "v.cv", 0);
   a.level = level;
   a.fail = fail;
   a.inv_lam = inv_lam;
@@ -2287,8 +2337,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.need_grad = need_grad;
   a.nu_code = nu_code;
   a.var = std::exp(theta[0]);
-  a.nug = std::exp(theta[1 + d]);
+  a.nug = std::exp(theta[P - 1]);
   a.rec_stride = NC;
+  a.gnu = svh_bessel::matern_nu(nu_code == 3 ? nuv : 2.5);
+  a.gnu_p = svh_bessel::matern_nu(nu_code == 3 ? nuv + kNuStep : 2.5);
+  a.gnu_m = svh_bessel::matern_nu(nu_code == 3 ? nuv - kNuStep : 2.5);
+  a.est_nu = est_nu ? 1 : 0;
   a.prof = nullptr;
   a.zmode = vs.zmode;
   a.sig32 = precision == SVH_MIXED_F32COV ? 1 : 0;

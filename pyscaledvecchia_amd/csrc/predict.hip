@@ -31,6 +31,7 @@
 // the K real diagonal entries.
 
 #include "svh_internal.h"
+#include "bessel.h"
 
 #include <cmath>
 #include <cstdlib>
@@ -93,9 +94,12 @@ struct PredArgs {
   int64_t nblk;
   int nb_stride, tg_stride, noise_free, chunk_step, d, K, nu_code, mode;
   double var, nug;
+  svh_bessel::MaternNu gnu;   // nu_code 3: general smoothness (Bessel K_nu)
 };
 
-__device__ __forceinline__ double matern_M(int nu_code, double r) {
+__device__ __forceinline__ double matern_M(int nu_code, double r,
+                                           const svh_bessel::MaternNu& gnu) {
+  if (nu_code == 3) return svh_bessel::matern_general_M(gnu, r);
   if (nu_code == 0) return exp(-r);
   if (nu_code == 1) {
     double c = 1.7320508075688772 * r;
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(kWave) void pred_blocks_kernel(PredArgs a) {
         double t = xs[i * d + l] - xs[j * d + l];
         s += t * t;
       }
-      double M = (i == j) ? 1.0 : matern_M(a.nu_code, sqrt(s));
+      double M = (i == j) ? 1.0 : matern_M(a.nu_code, sqrt(s), a.gnu);
       double nug = (i == j) ? a.nug * msk[i] : 0.0;
       Lp[p] = a.var * (M + nug);
     }
@@ -310,7 +314,7 @@ __global__ __launch_bounds__(kPT) void pred_mfma_kernel(PredArgs a, PredTileArgs
               const double t = xs[row * d + l] - xs[col * d + l];
               s2 = fma(t, t, s2);
             }
-            val = a.var * matern_M(a.nu_code, sqrt(s2));
+            val = a.var * matern_M(a.nu_code, sqrt(s2), a.gnu);
           }
         } else if (aug && row == Kb) {
           val = (col < Kb - 1) ? rv[col] : (col == Kb ? kInf : 0.0);
@@ -504,8 +508,10 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
     nu_code = 1;
   else if (nu == 2.5)
     nu_code = 2;
+  else if (nu > 0.0 && std::isfinite(nu))
+    nu_code = 3;
   else
-    throw SvhError{SVH_ERR_ARG, "predict: nu must be 0.5, 1.5 or 2.5"};
+    throw SvhError{SVH_ERR_ARG, "predict: nu must be positive"};
   if (nblk == 0) return;
   hipStream_t s = ctx->stream;
   // tiled matrix-core kernel while K (+1 appended row) fits 9 tiles of 16
@@ -557,6 +563,7 @@ void predict_blocks_device(svh_ctx* ctx, const double* dXn, const int64_t* dnb, 
   a.d = d;
   a.K = K;
   a.nu_code = nu_code;
+  a.gnu = svh_bessel::matern_nu(nu_code == 3 ? nu : 2.5);
   a.mode = mode;
   a.var = std::exp(theta[0]);
   a.nug = std::exp(theta[1 + d]);

@@ -107,7 +107,7 @@ struct VecchiaState {
   bool ready = false;
   int64_t n = 0, nblk = 0;
   int d = 0, q = 0, mmax = 0, kp = 0;
-  int nchunk_grad = 0, nchunk_val = 0;
+  int nchunk_grad = 0, nchunk_val = 0, nchunk_grad_nu = 0;
   int zmode = 0;   // trend columns formed in the kernels: 0 none, 1 [1, x], 2 constant
   // device pointers live in the arena under "v.*" names
 };

@@ -36,7 +36,9 @@ class ScaledVecchiaGP:
 
     m_est / m_pred : conditioning-set sizes for estimation / prediction
     n_est          : subsample size used for estimation
-    nu             : Matern smoothness, one of 0.5, 1.5, 2.5
+    nu             : Matern smoothness: any positive value (closed forms at
+                     0.5, 1.5, 2.5, the Bessel form otherwise), or None to
+                     estimate it as an extra parameter
     trend          : 'zero', 'constant' or 'linear'
     nugget         : None -> estimated relative nugget; float -> fixed
     var_correction : estimate the predictive-variance factor b (Sec. 3.4)
@@ -116,6 +118,9 @@ class ScaledVecchiaGP:
     def _clip_theta(self, theta, d, estimate_nu=False):
         hi = math.log(self.lambda_max) if np.isfinite(self.lambda_max) else np.inf
         theta[1:1 + d] = np.clip(theta[1:1 + d], math.log(1e-8), hi)
+        if estimate_nu:
+            # smoothness kept where the Bessel evaluations are well conditioned
+            theta[1 + d] = np.clip(theta[1 + d], math.log(0.3), math.log(10.0))
         theta[-1] = np.clip(theta[-1], math.log(1e-12), math.log(1e3))
         return theta
 
@@ -123,10 +128,8 @@ class ScaledVecchiaGP:
         return _lib.context(self.device)
 
     def _check_nu(self):
-        if self.nu is None or self.nu not in (0.5, 1.5, 2.5):
-            raise NotImplementedError(
-                "the GPU path implements the closed-form Matern nu in {0.5, 1.5, 2.5}; "
-                "general or estimated nu is not supported")
+        if self.nu is not None and not (np.isfinite(self.nu) and self.nu > 0):
+            raise ValueError("nu must be positive, or None to estimate it")
 
     # ---------------- fitting (gp.py:138-239) -----------------------------
     def _prepare(self, X, y):
@@ -140,7 +143,10 @@ class ScaledVecchiaGP:
             raise ValueError("X and y have incompatible shapes")
         self._design(X[:1])                    # validates `trend` early
         self._check_nu()
-        self._estimate_nu = False
+        est = self.nu is None                  # gp.py:145
+        self._estimate_nu = est
+        if est and self.warm_start is not None and self.theta0 is None:
+            raise ValueError("warm_start needs a fixed nu")
         self.d_ = d
         self._lo = X.min(0)
         self._span = np.where(X.max(0) - X.min(0) > 0, X.max(0) - X.min(0), 1.0)
@@ -159,15 +165,18 @@ class ScaledVecchiaGP:
         m = min(self.m_est, len(sub) - 1)
 
         fixed_nug = self.nugget is not None
-        default = np.empty(d + 2)
+        P = d + 2 + (1 if est else 0)
+        default = np.empty(P)
         default[0] = math.log(max(ye.var(), 1e-8))
         default[1:1 + d] = math.log(0.2 * math.sqrt(d))
+        if est:
+            default[1 + d] = math.log(1.5)     # gp.py:174-175
         default[-1] = math.log(self.nugget if fixed_nug else 0.1)
-        default = self._clip_theta(default, d)
+        default = self._clip_theta(default, d, est)
         if self.theta0 is not None:
             theta = np.array(self.theta0, dtype=float).copy()
-            if theta.shape != (d + 2,):
-                raise ValueError("theta0 must have d + 2 entries")
+            if theta.shape != (P,):
+                raise ValueError(f"theta0 must have {P} entries")
         elif self.warm_start is not None:
             theta = getattr(self, "_ws_theta0", None)
             if theta is None:
@@ -177,7 +186,7 @@ class ScaledVecchiaGP:
             theta = default.copy()
         if fixed_nug:
             theta[-1] = math.log(self.nugget)
-        theta = self._clip_theta(theta, d)
+        theta = self._clip_theta(theta, d, est)
         return dict(n=n, d=d, rng=rng, Xe=Xe, ye=ye, Ze=Ze, m=m, theta=theta,
                     theta_default=default, fixed_nug=fixed_nug,
                     lvt=math.log(max(ye.var(), 1e-8)))
@@ -202,13 +211,14 @@ class ScaledVecchiaGP:
     def _finish(self, theta, ll, beta, rng, n):
         """Final clip, fitted attributes and the b-step (gp.py:227-239)."""
         d = self.d_
-        theta = self._clip_theta(np.array(theta), d)
+        est = self._estimate_nu
+        theta = self._clip_theta(np.array(theta), d, est)
         if self.nugget is not None:
             theta[-1] = math.log(self.nugget)
         self.theta_ = theta
         self.beta_ = beta
         self.loglik_ = ll
-        self._nu_fit = self.nu
+        self._nu_fit = math.exp(theta[1 + d]) if est else self.nu     # gp.py:233
         self.b_ = 1.0
         if self.var_correction and n >= 200:
             self.b_ = self._estimate_b(rng)
@@ -227,6 +237,10 @@ class ScaledVecchiaGP:
         self.n_rebuilds_ = 0
         self.n_evals_ = 0
 
+        est = self._estimate_nu
+        # the smoothness argument of the evaluations: 0 when it is estimated
+        # (theta[1 + d] = log nu)
+        nu_arg = 0.0 if est else self.nu
         dev = DeviceRebuild(Xe, ye, Ze, m, ctx=ctx) if DeviceRebuild.supported(d, m) else None
 
         def rebuild(th):
@@ -239,7 +253,7 @@ class ScaledVecchiaGP:
                 groups = _nn_groups(nn, m)
                 state["order"] = order
                 state["struct"] = VecchiaStructure(Xe[order], ye[order], Ze[order], groups,
-                                                   ctx=ctx)
+                                                   ctx=ctx, estimate_nu=est)
             self.n_rebuilds_ += 1
 
         rebuild(theta)
@@ -247,11 +261,11 @@ class ScaledVecchiaGP:
         log_var_target = prep["lvt"]
 
         def objective(th, precision=self.precision):
-            th = self._clip_theta(np.array(th, dtype=float), d)
+            th = self._clip_theta(np.array(th, dtype=float), d, est)
             if fixed_nug:
                 th[-1] = math.log(self.nugget)
             self.n_evals_ += 1
-            ll, g, M, beta = state["struct"](th, self.nu, need_grad=True,
+            ll, g, M, beta = state["struct"](th, nu_arg, need_grad=True,
                                              var_penalty=self.var_penalty,
                                              log_var_target=log_var_target,
                                              precision=precision)
@@ -262,7 +276,7 @@ class ScaledVecchiaGP:
 
         def reorder_hook(th, k):
             nonlocal last_ranges
-            r = np.exp(self._clip_theta(np.array(th), d)[1:1 + d])
+            r = np.exp(self._clip_theta(np.array(th), d, est)[1:1 + d])
             if np.max(np.abs(np.log(r / last_ranges))) < 1e-3:
                 return False
             if self.verbose:
@@ -305,7 +319,8 @@ class ScaledVecchiaGP:
         other."""
         from dataclasses import replace
         d = np.atleast_2d(X).shape[1]
-        if d <= 10 and self._design(np.zeros((1, d))).shape[1] <= 11 and self.m_est <= 31:
+        if (d <= 10 and self._design(np.zeros((1, d))).shape[1] <= 11 and self.m_est <= 31
+                and self.nu in (0.5, 1.5, 2.5)):
             from .batch import BatchedFit
             BatchedFit([self], device=self.device).fit([X], [y])
             return self
@@ -331,7 +346,7 @@ class ScaledVecchiaGP:
         # FP64 blocks whatever the fit's precision: with tiny nuggets the
         # K = m + 1 blocks are ill-conditioned and FP32 trailing updates move
         # the predictive variance (and the b correction) visibly
-        return self._ctx().predict(X_tr, y_tr_resid, Xs_std, m, self.theta_, self.nu_,
+        return self._ctx().predict(X_tr, y_tr_resid, Xs_std, m, self._theta_cov(), self.nu_,
                                    noise_free=noise_free, precision="fp64")
 
     def predict(self, Xstar, return_std=False, return_var=False, m=None):
@@ -382,7 +397,7 @@ class ScaledVecchiaGP:
         mask = np.ones((ns, K))
         mask[:, K - 1] = 0.0
         mask[:, :K - 1] = (block[:, :K - 1] < n).astype(float)
-        av = self._ctx().block_columns(Xo, block, mask, self.theta_, self.nu_,
+        av = self._ctx().block_columns(Xo, block, mask, self._theta_cov(), self.nu_,
                                        precision="fp64")
         # U's column i holds av[i] at rows block[i]; the solves with
         # Lt = U_pp^T run on the device (svh_joint_solve)
@@ -442,17 +457,27 @@ class ScaledVecchiaGP:
         b = float(np.mean((self.y_[te] - mu) ** 2 / va))
         return max(b, 1e-8)
 
+    def _theta_cov(self):
+        """[log sigma^2, log lambda, log tau]: the parameters of the prediction
+        blocks (the fitted smoothness goes separately, as nu_)."""
+        th = self.theta_
+        if self._estimate_nu:
+            th = np.concatenate([th[:1 + self.d_], th[-1:]])
+        return th
+
     def _check_fitted(self):
         if self.theta_ is None:
             raise RuntimeError("call .fit(X, y) first")
 
     def summary(self):
         self._check_fitted()
-        s = [f"ScaledVecchiaGP(nu={self.nu}, m_est={self.m_est}, m_pred={self.m_pred}, "
+        nu_label = "estimated" if self._estimate_nu else "fixed"
+        s = [f"ScaledVecchiaGP(nu={'estimate' if self.nu is None else self.nu}, "
+             f"m_est={self.m_est}, m_pred={self.m_pred}, "
              f"trend='{self.trend}')",
              f"  Vecchia loglik (standardised y) : {self.loglik_:.3f}",
              f"  sigma^2 : {self.variance_:.6g}",
-             f"  nu      : {self.nu_:.4g}  (fixed)",
+             f"  nu      : {self.nu_:.4g}  ({nu_label})",
              f"  nugget  : {self.nugget_:.6g}  (relative)",
              f"  b       : {self.b_:.4g}  (variance correction)",
              "  input relevances 1/lambda_l (unit-box scaling):"]

@@ -47,7 +47,7 @@ class VecchiaStructure:
     """Device-resident Vecchia structure: ordered inputs, responses, design
     and neighbour groups (the state gp.py:182-191 rebuilds)."""
 
-    def __init__(self, Xo, yo, Zo, groups, ctx=None):
+    def __init__(self, Xo, yo, Zo, groups, ctx=None, estimate_nu=False):
         self.ctx = ctx or _lib.context()
         Xo = np.ascontiguousarray(Xo, dtype=np.float64)
         yo = np.ascontiguousarray(yo, dtype=np.float64)
@@ -67,7 +67,9 @@ class VecchiaStructure:
             nbr[o:o + B, :k] = nb
             kcnt[o:o + B] = k
             o += B
-        P = self.d + 2
+        # jitter chunks of the gradient pass use P = d + 2, or d + 3 when the
+        # smoothness is an extra parameter (likelihood.py:93, 114)
+        P = self.d + (3 if estimate_nu else 2)
         cg, ncg = _chunk_ids(groups, P)
         cv, ncv = _chunk_ids(groups, 1)
         self.ctx.vecchia_setup(Xo, yo, None if q == 0 else Zo, rows, nbr, kcnt, max(mmax, 1),
@@ -117,20 +119,21 @@ class DeviceRebuild:
 
 
 def _check_nu(nu):
-    if nu not in (0.5, 1.5, 2.5):
-        raise NotImplementedError(
-            "the GPU path implements the closed-form Matern nu in {0.5, 1.5, 2.5}")
+    if not (nu is not None and np.isfinite(nu) and nu > 0):
+        raise ValueError("nu must be positive (closed forms at 0.5, 1.5, 2.5, the Bessel "
+                         "form otherwise)")
 
 
 def vecchia_profile_loglik(theta, Xo, yo, Zo, groups, nu, estimate_nu=False,
                            need_grad=True, var_penalty=0.0, log_var_target=0.0,
                            precision="fp64"):
     """Profile Vecchia loglikelihood, gradient and Fisher information
-    (likelihood.py:66-175).  Returns (ll, grad, fisher, beta), or
-    (ll, None, None, beta) when need_grad is False."""
-    if estimate_nu:
-        raise NotImplementedError("estimating nu (Bessel Matern) is not on the GPU path")
-    _check_nu(nu)
-    st = VecchiaStructure(Xo, yo, Zo, groups)
-    return st(theta, nu, need_grad=need_grad, var_penalty=var_penalty,
-              log_var_target=log_var_target, precision=precision)
+    (likelihood.py:66-175).  theta = [log sigma^2, log lambda_1..d, log tau]
+    (P = d + 2), or [log sigma^2, log lambda_1..d, log nu, log tau] (P = d + 3)
+    with estimate_nu, where `nu` is then ignored.  Returns (ll, grad, fisher,
+    beta), or (ll, None, None, beta) when need_grad is False."""
+    if not estimate_nu:
+        _check_nu(nu)
+    st = VecchiaStructure(Xo, yo, Zo, groups, estimate_nu=estimate_nu)
+    return st(theta, 0.0 if estimate_nu else nu, need_grad=need_grad,
+              var_penalty=var_penalty, log_var_target=log_var_target, precision=precision)

@@ -109,5 +109,8 @@ def test_estimator_validates_before_device_use():
         ScaledVecchiaGP(trend="bogus").fit(np.random.rand(20, 2), np.random.rand(20))
     with pytest.raises(RuntimeError):
         ScaledVecchiaGP().predict(np.zeros((1, 2)))
-    with pytest.raises(NotImplementedError):
-        ScaledVecchiaGP(nu=1.75).fit(np.random.rand(20, 2), np.random.rand(20))
+    with pytest.raises(ValueError):
+        ScaledVecchiaGP(nu=-1.0).fit(np.random.rand(20, 2), np.random.rand(20))
+    with pytest.raises(ValueError):
+        ScaledVecchiaGP(nu=None, warm_start="learned").fit(np.random.rand(20, 2),
+                                                           np.random.rand(20))
