@@ -81,9 +81,11 @@ def run_c3(args):
     data = c3_datasets(args.ndata)
     kw = dict(m_est=30, n_est=5000, trend="constant", var_correction=False)
     ctx = _lib.context()
-    # warm-up (kernels, allocations)
-    BatchedFit([ScaledVecchiaGP(nu=nu, **kw) for _, _, nu in data[:2]]).fit(
-        [X for X, _, _ in data[:2]], [y for _, y, _ in data[:2]])
+    if not args.no_autotune:
+        ctx.autotune(2)       # the batched pass 1's tile size, tuned by the warm-up batch
+    # warm-up (kernels, allocations, the tile size of this batch shape)
+    BatchedFit([ScaledVecchiaGP(nu=nu, **kw) for _, _, nu in data]).fit(
+        [X for X, _, _ in data], [y for _, y, _ in data])
     models = [ScaledVecchiaGP(nu=nu, **kw) for _, _, nu in data]
     ctx.timing(True)
     ctx.timing_reset()
@@ -98,7 +100,9 @@ def run_c3(args):
            "dtype": "f64", "config": {"workload": "C3 64 uneven datasets, m_est=30, n_est=5000",
                                       "datasets": len(data)},
            "batch_s": t_batch, "stats": bf.stats,
-           "graph_step_ms_avg": st_ms / max(st_n, 1), "graph_steps": st_n}
+           "graph_step_ms_avg": st_ms / max(st_n, 1), "graph_steps": st_n,
+           "autotune": {k: {"candidate": v[0], "ms": v[1]}
+                        for k, v in ctx.autotune_choices().items()}}
     if not args.no_seq:
         t0 = time.perf_counter()
         seq = [ScaledVecchiaGP(nu=nu, **kw).fit(X, y) for X, y, nu in data]

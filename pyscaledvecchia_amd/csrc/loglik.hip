@@ -1572,69 +1572,31 @@ struct BatchArgs {
   const int* level;
   int* fail;
   double* cache;
-  double* partial;   // (ndata, GX, kBNACC)
-  int mmax_stride, GX;
+  double* partial;   // (tiles, kBNACC)
+  int mmax_stride;
 };
 
-__global__ __launch_bounds__(kWave, 2) void ll_pass1_batch(BatchArgs B) {
-  extern __shared__ double lds[];
-  const int i = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
-  if (!fs_evaluates(B.fs[i].status)) return;
-  const DSDesc D = B.ds[i];
-  if (x >= D.nblk) return;   // small datasets leave spare workgroups idle
-  LLArgs a;
-  a.X = B.X + D.xoff;
-  a.y = B.y + D.yoff;
-  a.Z = B.Z + D.zoff;
-  a.rows = B.rows;
-  a.nbr = B.nbr;
-  a.kcnt = B.kcnt;
-  a.chunk = B.chunk;
-  a.level = B.level;
-  a.fail = B.fail;
-  a.inv_lam = B.par + i * 16 + 2;
-  a.cache = B.cache;
-  a.partial = nullptr;
-  a.nblk = D.blk0 + D.nblk;
-  a.d = D.d;
-  a.q = D.q;
-  a.mmax = B.mmax_stride;
-  a.P = D.P;
-  a.need_grad = 1;
-  a.nu_code = D.nu_code;
-  a.var = B.par[i * 16 + 0];
-  a.nug = B.par[i * 16 + 1];
-  a.rec_stride = kBNC;
-  double accF[4] = {0.0, 0.0, 0.0, 0.0};
-  double accX[3] = {0.0, 0.0, 0.0};
-  unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  a.prof = nullptr;
-  a.zmode = 0;
-  a.sig32 = 0;
-  a.est_nu = 0;
-  for (int64_t b = D.blk0 + x; b < D.blk0 + D.nblk; b += B.GX)
-    split_block<-1, false, false>(a, b, lds, accF, accX, ph);
-  const int P = D.P, q = D.q;
-  const int nX = q * q + q;
-  double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
-  write_fisher<false>(part, accF, P, lane);
-  for (int e = lane; e < nX; e += kWave)
-    part[P * P + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
-}
+// Work list of the batched pass 1: tile = (dataset, first pair, end pair).
+// Tiles hold a fixed number of block pairs, so a dataset gets workgroups in
+// proportion to its size; tiles are grouped by the coordinate registers
+// their dataset needs (DM = 4, 8, 10), one launch per group.
+struct BTile {
+  int ds, p0, p1, pad;
+};
 
-// The rows-in-lanes pass 1 for the batch: dataset i's blocks in pairs (the
-// pairs start at the dataset's first block), nu at run time, coordinates in DM
-// >= every dataset's d registers.
+// The rows-in-lanes pass 1 over one tile: pairs p0..p1-1 of dataset ds (the
+// pairs start at the dataset's first block), nu at run time.
 template <int DM>
-__global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
+__global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_tiles(BatchArgs B,
+                                                                const BTile* __restrict__ tiles,
+                                                                int tile0) {
   extern __shared__ double lds[];
-  const int i = blockIdx.y, x = blockIdx.x, lane = threadIdx.x;
+  const int lane = threadIdx.x;
+  const int tix = tile0 + blockIdx.x;
+  const BTile t = tiles[tix];
+  const int i = t.ds;
   if (!fs_evaluates(B.fs[i].status)) return;
   const DSDesc D = B.ds[i];
-  const int64_t npair = (D.nblk + 1) / 2;
-  // the partial sums read min(nblk, GX) workgroups per dataset: those past the
-  // last pair still write (zero) partials
-  if (x >= D.nblk) return;
   LLArgs a;
   a.X = B.X + D.xoff;
   a.y = B.y + D.yoff;
@@ -1665,11 +1627,11 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t pp = x; pp < npair; pp += B.GX)
-    rows_pair<-1, false, DM, false>(a, D.blk0 + 2 * pp, lds, accF, accX, ph);
+  for (int pp = t.p0; pp < t.p1; ++pp)
+    rows_pair<-1, false, DM, false>(a, D.blk0 + 2 * (int64_t)pp, lds, accF, accX, ph);
   const int P = D.P, q = D.q;
   const int nX = q * q + q;
-  double* part = B.partial + ((int64_t)i * B.GX + x) * kBNACC;
+  double* part = B.partial + (int64_t)tix * kBNACC;
   write_fisher<false>(part, accF, P, lane);
   for (int e = lane; e < nX; e += kWave)
     part[P * P + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
@@ -1678,17 +1640,17 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_rows(BatchArgs B) {
 // (entry, dataset) workgroups: fixed-order sums of the pass-1 partials
 __global__ __launch_bounds__(256) void ll_sum_batch(const DSDesc* __restrict__ ds,
                                                     const FSState* __restrict__ fs,
-                                                    const double* __restrict__ partial, int GX,
+                                                    const double* __restrict__ partial,
+                                                    const int* __restrict__ dtile,
                                                     double* __restrict__ tot) {
   __shared__ double red[256];
   const int e = blockIdx.x, i = blockIdx.y;
   if (!fs_evaluates(fs[i].status)) return;
   const DSDesc D = ds[i];
   if (e >= D.P * D.P + D.q * D.q + D.q) return;
-  const int gx = D.nblk < GX ? (int)D.nblk : GX;
+  const int t0 = dtile[2 * i], nt = dtile[2 * i + 1];   // the dataset's tiles
   double s = 0.0;
-  for (int x = threadIdx.x; x < gx; x += 256)
-    s += partial[((int64_t)i * GX + x) * kBNACC + e];
+  for (int x = threadIdx.x; x < nt; x += 256) s += partial[(int64_t)(t0 + x) * kBNACC + e];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
@@ -2474,9 +2436,20 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
 
 // ---------------------------------------------------------------- batched host side
 
+// Pairs per tile of the batched pass 1: the autotuner's candidates.
+constexpr int kTileCand[6] = {2, 4, 8, 16, 32, 64};
+constexpr int kNTileCand = 6;
+
 struct BatchState {
   bool ready = false;
-  int ndata = 0, dmax = 0, GX = 0, G2X = 0, mmax = 0, nchunk = 0;
+  int ndata = 0, dmax = 0, G2X = 0, mmax = 0, nchunk = 0;
+  // tile layouts, one per candidate: tiles of the DM = 4 / 8 / 10 groups at
+  // toff[c] + goff[c][g] (goff[c][3] = total), per-dataset (first, count) at
+  // dtile + c * 2 * ndata
+  int64_t toff[kNTileCand] = {};
+  int goff[kNTileCand][4] = {};
+  int cand = 0;
+  bool tuned = false;
   int64_t nblk = 0, nx = 0, ny = 0, nz = 0;
   std::vector<DSDesc> desc;
   hipGraph_t graph = nullptr;
@@ -2555,9 +2528,41 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   // few datasets enough for two waves per SIMD of the chip (8 one-wave
   // workgroups per CU in pass 1, as the single-dataset launch uses)
   const int ncu = ctx->num_cu > 0 ? ctx->num_cu : 256;
-  B.GX = (int)std::min<int64_t>(maxn, std::max(1024, (8 * ncu + ndata - 1) / ndata));
   B.G2X = (int)std::min<int64_t>((maxn * kB2L + 255) / 256,
                                   std::max(64, (8 * ncu + ndata - 1) / ndata));
+  // tile layouts for every candidate tile size; the default is the candidate
+  // whose tile count is nearest 8 workgroups per CU (two waves per SIMD)
+  std::vector<BTile> htiles;
+  std::vector<int> hdtile((size_t)kNTileCand * 2 * ndata);
+  int64_t pairs_tot = 0;
+  for (auto& D : B.desc) pairs_tot += (D.nblk + 1) / 2;
+  double best_gap = 1e300;
+  for (int c = 0; c < kNTileCand; ++c) {
+    const int T = kTileCand[c];
+    B.toff[c] = (int64_t)htiles.size();
+    for (int g = 0; g < 3; ++g) {
+      B.goff[c][g] = (int)((int64_t)htiles.size() - B.toff[c]);
+      for (int i = 0; i < ndata; ++i) {
+        const DSDesc& D = B.desc[i];
+        const int gi = D.d <= 4 ? 0 : (D.d <= 8 ? 1 : 2);
+        if (gi != g) continue;
+        const int np = (int)((D.nblk + 1) / 2);
+        hdtile[((size_t)c * ndata + i) * 2] = (int)((int64_t)htiles.size() - B.toff[c]);
+        int cnt = 0;
+        for (int p0 = 0; p0 < np; p0 += T, ++cnt) htiles.push_back({i, p0, std::min(np, p0 + T), 0});
+        hdtile[((size_t)c * ndata + i) * 2 + 1] = cnt;
+      }
+    }
+    B.goff[c][3] = (int)((int64_t)htiles.size() - B.toff[c]);
+    const double gap = std::fabs(std::log((double)B.goff[c][3] / (8.0 * ncu)));
+    if (gap < best_gap) {
+      best_gap = gap;
+      B.cand = c;
+    }
+  }
+  B.tuned = false;
+  int max_tiles = 0;
+  for (int c = 0; c < kNTileCand; ++c) max_tiles = std::max(max_tiles, B.goff[c][3]);
   // block tables (rows are 0..n-1 within each dataset, in order)
   // (nbr / kcnt NULL: empty lists, filled later by device rebuilds)
   const bool lists = nbr != nullptr;
@@ -2610,7 +2615,9 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   ctx->arena.get("b.level", (size_t)co * sizeof(int));
   ctx->arena.get("b.fail", (size_t)co * sizeof(int));
   ctx->arena.get("b.cache", (size_t)bo * kBNC * sizeof(double));
-  ctx->arena.get("b.partial", (size_t)ndata * B.GX * kBNACC * sizeof(double));
+  up("b.tiles", htiles.data(), htiles.size() * sizeof(BTile));
+  up("b.dtile", hdtile.data(), hdtile.size() * sizeof(int));
+  ctx->arena.get("b.partial", (size_t)std::max(max_tiles, 1) * kBNACC * sizeof(double));
   ctx->arena.get("b.tot", (size_t)ndata * kBNACC * sizeof(double));
   ctx->arena.get("b.beta", (size_t)ndata * kBQ * sizeof(double));
   ctx->arena.get("b.partial2", (size_t)ndata * B.G2X * kBNV * sizeof(double));
@@ -2679,8 +2686,31 @@ BatchLayout batch_layout(svh_ctx* ctx, int i) {
   return L;
 }
 
-// Enqueue one evaluation of every evaluating dataset plus the state update.
-static void batch_enqueue(svh_ctx* ctx, bool with_update) {
+// Pass 1 of every evaluating dataset with the tile layout of candidate c:
+// one launch per coordinate-register group (DM = 4, 8, 10).
+static void batch_pass1(svh_ctx* ctx, const BatchArgs& a, int c) {
+  BatchState& B = batch_of(ctx);
+  const BTile* tiles = (const BTile*)ctx->arena.get("b.tiles", 0) + B.toff[c];
+  for (int g = 0; g < 3; ++g) {
+    const int t0 = B.goff[c][g], nt = B.goff[c][g + 1] - t0;
+    if (nt <= 0) continue;
+    const int dm = g == 0 ? 4 : (g == 1 ? 8 : 10);
+    const size_t lds = (size_t)rows_lds_doubles(dm) * sizeof(double);
+    if (g == 0)
+      hipLaunchKernelGGL(ll_pass1_batch_tiles<4>, dim3(nt), dim3(kWave), lds, ctx->stream, a,
+                         tiles, t0);
+    else if (g == 1)
+      hipLaunchKernelGGL(ll_pass1_batch_tiles<8>, dim3(nt), dim3(kWave), lds, ctx->stream, a,
+                         tiles, t0);
+    else
+      hipLaunchKernelGGL(ll_pass1_batch_tiles<10>, dim3(nt), dim3(kWave), lds, ctx->stream, a,
+                         tiles, t0);
+  }
+}
+
+// Parameters of the current evaluation point per dataset, chunk levels reset,
+// and the pass-1 arguments.
+static BatchArgs batch_prelude(svh_ctx* ctx) {
   BatchState& B = batch_of(ctx);
   hipStream_t s = ctx->stream;
   const DSDesc* dd = (const DSDesc*)ctx->arena.get("b.desc", 0);
@@ -2690,13 +2720,6 @@ static void batch_enqueue(svh_ctx* ctx, bool with_update) {
   int* level = (int*)ctx->arena.get("b.level", 0);
   int* fail = (int*)ctx->arena.get("b.fail", 0);
   const int* chunk_ds = (const int*)ctx->arena.get("b.chunk_ds", 0);
-  double* out = (double*)ctx->arena.get("b.out", 0);
-  double* beta = (double*)ctx->arena.get("b.beta", 0);
-  double* tot = (double*)ctx->arena.get("b.tot", 0);
-  double* partial = (double*)ctx->arena.get("b.partial", 0);
-  double* partial2 = (double*)ctx->arena.get("b.partial2", 0);
-  double* cache = (double*)ctx->arena.get("b.cache", 0);
-  int* status = (int*)ctx->arena.get("b.status", 0);
   hipLaunchKernelGGL(fs_params, dim3((B.ndata + 63) / 64), dim3(64), 0, s, dd, fs, par, the,
                      B.ndata);
   hipLaunchKernelGGL(fs_reset_chunks, dim3((B.nchunk + 255) / 256), dim3(256), 0, s, chunk_ds, fs,
@@ -2714,26 +2737,55 @@ static void batch_enqueue(svh_ctx* ctx, bool with_update) {
   a.chunk = (const int*)ctx->arena.get("b.chunk", 0);
   a.level = level;
   a.fail = fail;
-  a.cache = cache;
-  a.partial = partial;
+  a.cache = (double*)ctx->arena.get("b.cache", 0);
+  a.partial = (double*)ctx->arena.get("b.partial", 0);
   a.mmax_stride = B.mmax;
-  a.GX = B.GX;
-  // pass 1: the rows-in-lanes kernel (SVH_BATCH_KERNEL=split: the split-row one)
-  const char* bk = std::getenv("SVH_BATCH_KERNEL");
-  if (bk && std::string(bk) == "split") {
-    const size_t lds = (size_t)split_lds_doubles(B.dmax) * sizeof(double);
-    hipLaunchKernelGGL(ll_pass1_batch, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
-  } else {
-    const int dm = B.dmax <= 4 ? 4 : (B.dmax <= 8 ? 8 : 10);
-    const size_t lds = (size_t)rows_lds_doubles(dm) * sizeof(double);
-    if (dm == 4)
-      hipLaunchKernelGGL(ll_pass1_batch_rows<4>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
-    else if (dm == 8)
-      hipLaunchKernelGGL(ll_pass1_batch_rows<8>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
-    else
-      hipLaunchKernelGGL(ll_pass1_batch_rows<10>, dim3(B.GX, B.ndata), dim3(kWave), lds, s, a);
-  }
-  hipLaunchKernelGGL(ll_sum_batch, dim3(kBNACC, B.ndata), dim3(256), 0, s, dd, fs, partial, B.GX,
+  return a;
+}
+
+// The autotuner over the batched pass 1: the tile size (pairs per
+// workgroup) of the current batch, timed on its first evaluation point.
+static void batch_tune(svh_ctx* ctx) {
+  BatchState& B = batch_of(ctx);
+  if (B.tuned || ctx->tune.mode == 0) return;
+  B.tuned = true;
+  struct TLaunch {
+    svh_ctx* ctx;
+    static void run(void* u, int c) {
+      svh_ctx* cx = static_cast<TLaunch*>(u)->ctx;
+      const BatchArgs a = batch_prelude(cx);
+      batch_pass1(cx, a, c);
+    }
+  } lau{ctx};
+  int64_t pairs = 0;
+  for (auto& D : B.desc) pairs += (D.nblk + 1) / 2;
+  int lg = 0;
+  while ((int64_t(1) << (lg + 1)) <= pairs) ++lg;
+  char key[160];
+  std::snprintf(key, sizeof key, "batch_pass1:nd%d:dm%d:m%d:np2^%d", B.ndata, B.dmax, B.mmax,
+                lg);
+  B.cand = tune_choose(ctx, key, kNTileCand, B.cand, &TLaunch::run, &lau);
+}
+
+// Enqueue one evaluation of every evaluating dataset plus the state update.
+static void batch_enqueue(svh_ctx* ctx, bool with_update) {
+  BatchState& B = batch_of(ctx);
+  hipStream_t s = ctx->stream;
+  const DSDesc* dd = (const DSDesc*)ctx->arena.get("b.desc", 0);
+  FSState* fs = (FSState*)ctx->arena.get("b.fs", 0);
+  int* level = (int*)ctx->arena.get("b.level", 0);
+  int* fail = (int*)ctx->arena.get("b.fail", 0);
+  double* out = (double*)ctx->arena.get("b.out", 0);
+  double* beta = (double*)ctx->arena.get("b.beta", 0);
+  double* tot = (double*)ctx->arena.get("b.tot", 0);
+  double* partial = (double*)ctx->arena.get("b.partial", 0);
+  double* partial2 = (double*)ctx->arena.get("b.partial2", 0);
+  double* cache = (double*)ctx->arena.get("b.cache", 0);
+  int* status = (int*)ctx->arena.get("b.status", 0);
+  const BatchArgs a = batch_prelude(ctx);
+  batch_pass1(ctx, a, B.cand);
+  const int* dtile = (const int*)ctx->arena.get("b.dtile", 0) + (size_t)B.cand * 2 * B.ndata;
+  hipLaunchKernelGGL(ll_sum_batch, dim3(kBNACC, B.ndata), dim3(256), 0, s, dd, fs, partial, dtile,
                      tot);
   hipLaunchKernelGGL(ll_beta_batch, dim3(B.ndata), dim3(256), 0, s, dd, fs, tot, beta, out);
   hipLaunchKernelGGL(ll_pass2_batch, dim3(B.G2X, B.ndata), dim3(256), 0, s, dd, fs, cache, beta,
@@ -2773,6 +2825,7 @@ void batch_fit_start_device(svh_ctx* ctx, const double* theta0, int max_iter, do
     B.exec = nullptr;
     B.graph = nullptr;
   }
+  batch_tune(ctx);   // the tile size is fixed before the step is captured
   if (use_graph && !B.exec) {
     B.graph_var_penalty = var_penalty;
     hipStream_t s = ctx->stream;
