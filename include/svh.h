@@ -111,18 +111,6 @@ int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const do
                       const int32_t* chunk_grad, const int32_t* chunk_val, int32_t nchunk_grad,
                       int32_t nchunk_val);
 
-/* One profile Vecchia log-likelihood evaluation on the uploaded structure.
- *   theta (P = d+2): log sigma^2, log lambda_1..d, log tau, for a fixed
- *   smoothness nu > 0 (closed forms at 0.5, 1.5, 2.5; any other value uses
- *   the Bessel form of _matern_general.py:92-116).  nu == 0: the smoothness
- *   is estimated, theta (P = d+3) = log sigma^2, log lambda_1..d, log nu,
- *   log tau, and the log-nu derivative is a central difference in nu with
- *   step 1e-3 (_matern_general.py:134-179); the structure's gradient jitter
- *   chunks must then follow P = d+3 (svh_vecchia_rebuild provides both).
- *   Outputs: ll, grad[P], fisher[P*P] (row-major), beta[q]; grad/fisher are
- *   untouched when need_grad == 0.  *jitter_power_max receives the largest
- *   jitter power used by any chunk (0 = none, else -12..-4).
- * Replaces likelihood.py:66-175 (vecchia_profile_loglik), fixed or estimated nu. */
 /* The whole rebuild of a fit at new ranges, on the device: the estimation
  * data Xe (n x d), ye (n), Ze (n x q, NULL when q == 0) -- uploaded when
  * upload_data is nonzero, otherwise the copy of the previous call with the
@@ -140,6 +128,18 @@ int svh_vecchia_rebuild(svh_ctx* ctx, const double* Xe, const double* ye, const 
                         int64_t n, int32_t d, int32_t q, const double* lam, int64_t first,
                         int32_t m, int32_t upload_data, int64_t* order);
 
+/* One profile Vecchia log-likelihood evaluation on the uploaded structure.
+ *   theta (P = d+2): log sigma^2, log lambda_1..d, log tau, for a fixed
+ *   smoothness nu > 0 (closed forms at 0.5, 1.5, 2.5; any other value uses
+ *   the Bessel form of _matern_general.py:92-116).  nu == 0: the smoothness
+ *   is estimated, theta (P = d+3) = log sigma^2, log lambda_1..d, log nu,
+ *   log tau, and the log-nu derivative is a central difference in nu with
+ *   step 1e-3 (_matern_general.py:134-179); the structure's gradient jitter
+ *   chunks must then follow P = d+3 (svh_vecchia_rebuild provides both).
+ *   Outputs: ll, grad[P], fisher[P*P] (row-major), beta[q]; grad/fisher are
+ *   untouched when need_grad == 0.  *jitter_power_max receives the largest
+ *   jitter power used by any chunk (0 = none, else -12..-4).
+ * Replaces likelihood.py:66-175 (vecchia_profile_loglik), fixed or estimated nu. */
 int svh_vecchia_eval(svh_ctx* ctx, const double* theta, int32_t P, double nu,
                      int32_t precision, int32_t need_grad, double var_penalty,
                      double log_var_target, double* ll, double* grad, double* fisher,

@@ -55,6 +55,7 @@ void oracle_maximin(const double* X, int64_t n, int d, int64_t first, int64_t* o
 
 /* keep the k best (s, idx) in ascending order */
 static void insert_best(double* ks, int64_t* ki, int* cnt, int k, double s, int64_t j) {
+  if (k <= 0) return;   /* m = 0: nothing is kept (and ks[k - 1] does not exist) */
   int c = *cnt;
   if (c == k && !(s < ks[k - 1] || (s == ks[k - 1] && j < ki[k - 1]))) return;
   int p = (c < k) ? c : k - 1;
