@@ -316,7 +316,9 @@ int svh_autotune_read(svh_ctx* ctx, char* buf, int32_t len);
  * library brackets each launch of its dominant kernels with HIP events on the
  * context stream and accumulates their durations under a kernel name
  * ("vecchia_pass1", "predict_blocks", "knn", "maximin"; "maximin_rounds"
- * reads the number of selection rounds of the last maximin call as `count`).
+ * reads the number of selection rounds of the last maximin call as `count`,
+ * "maximin_fallbacks" how many of its rounds had more survivors than the
+ * selection holds and accepted only the largest candidate).
  * svh_timing_read returns the accumulated milliseconds and launch count. */
 int svh_timing_enable(svh_ctx* ctx, int32_t enable);
 int svh_timing_read(svh_ctx* ctx, const char* name, double* total_ms, int64_t* count);

@@ -496,6 +496,11 @@ int svh_timing_read(svh_ctx* ctx, const char* name, double* total_ms, int64_t* c
       *count = ctx->mm_rounds;
       return;
     }
+    if (std::string(name) == "maximin_fallbacks") {
+      *total_ms = 0.0;
+      *count = ctx->mm_fallbacks;
+      return;
+    }
     auto it = ctx->timers.acc.find(name);
     *total_ms = it == ctx->timers.acc.end() ? 0.0 : it->second.ms;
     *count = it == ctx->timers.acc.end() ? 0 : it->second.count;

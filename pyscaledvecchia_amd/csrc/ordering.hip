@@ -107,6 +107,7 @@ struct MMState {
   int nacc;   // accepted in the last round
   int rounds;
   int overflow_sorts;
+  int fallbacks;   // rounds that kept only the largest candidate (survivors > sortcap)
 };
 
 // One ordering problem of a batch (several datasets share each launch).
@@ -453,11 +454,9 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     }
   }
   block_argmax<NT>(Vk, Vi, rk, ri);
-  // survivors: first-T candidates strictly above V, compacted into LDS.
-  // Survivors that do not fit raise the bound instead (they leave the pool).
+  // survivors: first-T candidates strictly above V, compacted into LDS (the
+  // first sortcap of them; cnt counts all)
   int cnt = 0;
-  double Ok = -3.0;
-  int Oi = INT_MAX;
   const int ncand = nslice * T;
   for (int base = 0; base < ncand; base += NT) {
     const int c = base + tid;
@@ -481,24 +480,56 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
       tot += wcnt[w];
     }
     const int pos = cnt + woff + __popcll(m & ((1ull << lane) - 1ull));
-    if (keep) {
-      if (pos < sortcap) {
-        sk[pos] = k;
-        si[pos] = i;
-      } else if (mm_greater(k, i, Ok, Oi)) {
-        Ok = k;
-        Oi = i;
-      }
+    if (keep && pos < sortcap) {
+      sk[pos] = k;
+      si[pos] = i;
     }
     cnt += tot;
   }
-  block_argmax<NT>(Ok, Oi, rk, ri);
-  if (mm_greater(Ok, Oi, Vk, Vi)) {
-    Vk = Ok;
-    Vi = Oi;
-  }
   int nsurv = cnt < sortcap ? cnt : sortcap;
   __syncthreads();
+  if (cnt > sortcap) {
+    // More survivors than LDS holds: keep only the largest published
+    // candidate G.  G is the serial algorithm's next pick (every published
+    // candidate is at or below it, every other point at or below V < G); the
+    // largest of the remaining candidates becomes the bound, so the pool is
+    // {G} and the round accepts exactly G.  Exact, one pick per round.
+    double Gk = -3.0;
+    int Gi = INT_MAX;
+    for (int s = tid; s < nslice; s += NT) {
+      const double k = cand_key[s * (kSliceTop + 1)];
+      const int i = cand_idx[s * (kSliceTop + 1)];
+      if (mm_greater(k, i, Gk, Gi)) {
+        Gk = k;
+        Gi = i;
+      }
+    }
+    block_argmax<NT>(Gk, Gi, rk, ri);
+    const int gs = Gi / kSlice;   // G's slice (G is a survivor, so a point index)
+    double Sk = -3.0;
+    int Si = INT_MAX;
+    for (int s = tid; s < nslice; s += NT) {
+      const int e = (s == gs) ? 1 : 0;   // the next entry of G's own list
+      const double k = cand_key[s * (kSliceTop + 1) + e];
+      const int i = cand_idx[s * (kSliceTop + 1) + e];
+      if (mm_greater(k, i, Sk, Si)) {
+        Sk = k;
+        Si = i;
+      }
+    }
+    block_argmax<NT>(Sk, Si, rk, ri);
+    if (mm_greater(Sk, Si, Vk, Vi)) {
+      Vk = Sk;
+      Vi = Si;
+    }
+    if (tid == 0) {
+      sk[0] = Gk;
+      si[0] = Gi;
+      st->fallbacks += 1;
+    }
+    nsurv = 1;
+    __syncthreads();
+  }
   if (nsurv > NT) {
     // keep the top NT survivors: bitonic sort (descending) in LDS
     int L = 1;
@@ -743,6 +774,7 @@ __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __
     st.nacc = 1;
     st.rounds = 0;
     st.overflow_sorts = 0;
+    st.fallbacks = 0;
   }
 }
 
@@ -1145,6 +1177,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int sortcap = 1;
   while (sortcap < maxslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
+  // SVH_MM_SORTCAP (tests only): a smaller survivor capacity, so that the
+  // overflow path of mm_select runs at small n
+  if (const char* e = std::getenv("SVH_MM_SORTCAP")) {
+    const int v = std::atoi(e);
+    if (v >= kPool && v < sortcap && (v & (v - 1)) == 0) sortcap = v;
+  }
   const size_t select_lds =
       (size_t)sortcap * sizeof(double) + (size_t)((sortcap + 1) / 2) * sizeof(double);
   MMState* hst = static_cast<MMState*>(ctx->stage.get(nd * sizeof(MMState)));
@@ -1215,9 +1253,13 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   }
   SVH_HIP(hipStreamSynchronize(ctx->stream));
   ctx->timers.collect();
-  int rounds = 0;
-  for (int i = 0; i < nd; ++i) rounds = std::max(rounds, hst[i].rounds);
+  int rounds = 0, fallbacks = 0;
+  for (int i = 0; i < nd; ++i) {
+    rounds = std::max(rounds, hst[i].rounds);
+    fallbacks += hst[i].fallbacks;
+  }
   ctx->mm_rounds = rounds;
+  ctx->mm_fallbacks = fallbacks;
   if (mprof) {
     unsigned long long h[8];
     SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));

@@ -196,6 +196,7 @@ struct svh_ctx {
   VecchiaState vs;
   KernelTimers timers;
   int mm_rounds = 0;  // rounds of the last maximin call
+  int mm_fallbacks = 0;  // its rounds that kept only the largest candidate
   int64_t rb_n = 0;   // sizes of the estimation data resident for rebuilds
   int rb_d = 0, rb_q = 0, rb_zmode = 0;
   bool batch_data = false;   // unordered batch data uploaded for the current batch

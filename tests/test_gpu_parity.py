@@ -56,6 +56,33 @@ def test_maximin_large_prefix_and_property(gpu):
     assert np.all(np.diff(r) <= 1e-12)
 
 
+def _outlier_slices(nslice, seed):
+    """A tight cluster with four far outliers in every run of 256 consecutive
+    rows: after the first pick every 256-row slice publishes its outliers
+    above the bound, so a round has 4 * nslice survivors."""
+    rng = np.random.default_rng(seed)
+    X = rng.random((256 * nslice, 3)) * 0.1
+    for s in range(nslice):
+        rows = s * 256 + rng.choice(256, 4, replace=False)
+        X[rows] = rng.random((4, 3)) * 10.0
+    return X
+
+
+def test_maximin_survivor_overflow_stays_exact(gpu, monkeypatch):
+    """More survivors than the selection's capacity (forced small by
+    SVH_MM_SORTCAP): those rounds accept only the largest candidate, and
+    the order stays the serial one."""
+    from oracle import c_oracle as C
+    X = _outlier_slices(80, 21)
+    first = O.maximin_first(X)
+    ref = C.maximin_order(X, first) if C.available() else O.maximin_order(X, first)
+    np.testing.assert_array_equal(gpu.maximin(X, first), ref)
+    assert gpu.timing_read("maximin_fallbacks")[1] == 0
+    monkeypatch.setenv("SVH_MM_SORTCAP", "256")
+    np.testing.assert_array_equal(gpu.maximin(X, first), ref)
+    assert gpu.timing_read("maximin_fallbacks")[1] > 0
+
+
 @pytest.mark.parametrize("n,d,m,start,seed", [(120, 3, 8, 0, 2), (2000, 4, 30, 0, 7),
                                               (600, 8, 30, 400, 8), (60, 2, 6, 40, 4),
                                               (300, 1, 5, 0, 9), (200, 5, 140, 0, 10),
