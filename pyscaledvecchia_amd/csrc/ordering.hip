@@ -40,6 +40,34 @@ __device__ __forceinline__ bool nn_less(double ka, int ia, double kb, int ib) {
   return ka < kb || (ka == kb && ia < ib);
 }
 
+// Float32 screen for the maximin distance updates.  A point with exact
+// squared distance s to the chosen set is lowered by a new point only if
+// their canonical squared distance c is below s.  With the coordinates
+// rounded to float32 and S = sum fmaf(x - a, x - a) in float32:
+//   |S - t| <= 2^-20 t + 2^-21 ||m|| sqrt(t) + 2^-44 ||m||^2   (d <= 12),
+// where t is the exact real squared distance and m_l = max(|x_l|, |a_l|),
+// so ||m||^2 <= m2d = d * max|coordinate|^2; and |c - t| <= 2^-48 t.  Hence
+//   S >= T(s) = s (1 + 2^-14) + 2^-16 sqrt(m2d s) + 2^-38 m2d
+// implies c >= s (no lowering): only pairs with S < T(s) need the exact c.
+// Outside 1e-30 < T < 1e30 (float32 underflow / overflow) every pair is
+// checked exactly.
+__device__ __forceinline__ float screen_threshold(double s, double m2d) {
+  const double T = s * (1.0 + 0x1p-14) + 0x1p-16 * sqrt(m2d * s) + 0x1p-38 * m2d;
+  return (T > 1e-30 && T < 1e30) ? (float)T : INFINITY;
+}
+
+// float32 squared distance of the screen (contraction allowed: within the bound)
+template <int DM>
+__device__ __forceinline__ float screen_sq(const float (&x)[DM], const float* a) {
+  float t = 0.0f;
+#pragma unroll
+  for (int l = 0; l < DM; ++l) {
+    const float df = x[l] - a[l];
+    t = fmaf(df, df, t);
+  }
+  return t;
+}
+
 __device__ __forceinline__ void wave_argmax(double& k, int& i) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -108,6 +136,8 @@ struct MMState {
   int rounds;
   int overflow_sorts;
   int fallbacks;   // rounds that kept only the largest candidate (survivors > sortcap)
+  int pad;
+  double m2d;      // d * max |coordinate|^2 of the dataset (float32 screen bound)
 };
 
 // One ordering problem of a batch (several datasets share each launch).
@@ -226,7 +256,9 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     double* __restrict__ cand_keyall, int* __restrict__ cand_idxall) {
   constexpr int PW = kSlice / kWave;   // waves of part 0
   static_assert(kSliceTop + 1 <= 32, "published list exceeds half a wave");
+  static_assert(kAccCap <= 64 * kApplyParts, "screen mask holds 64 points per part");
   __shared__ double accX[kAccCap * DM];
+  __shared__ float accXf[kAccCap * DM];
   __shared__ int accI[kAccCap];
   __shared__ double pmin[kApplyParts - 1][kSlice];
   __shared__ int ppick[kApplyParts - 1][kSlice];
@@ -252,7 +284,9 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
   for (int t = tid; t < A; t += kSlice * kApplyParts) accI[t] = acc[t];
   for (int t = tid; t < A * DM; t += kSlice * kApplyParts) {
     const int a = t / DM;
-    accX[t] = accx[a * kMMD + (t - a * DM)];
+    const double v = accx[a * kMMD + (t - a * DM)];
+    accX[t] = v;
+    accXf[t] = (float)v;
   }
   __syncthreads();
   double di = (i < n) ? dist[i] : -1.0;
@@ -261,11 +295,26 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
   bool picked = false;
   if (active) {
     double xi[DM];
+    float xf[DM];
 #pragma unroll
-    for (int l = 0; l < DM; ++l) xi[l] = (l < d) ? XT[(int64_t)l * n + i] : 0.0;
+    for (int l = 0; l < DM; ++l) {
+      xi[l] = (l < d) ? XT[(int64_t)l * n + i] : 0.0;
+      xf[l] = (float)xi[l];
+    }
+    // float32 screen over this part's accepted points (bit k: point
+    // part + kApplyParts * k, at most 64 of them), then the exact canonical
+    // distance for the flagged ones only
+    const float thr = screen_threshold(s2[i], st->m2d);
+    unsigned long long flag = 0ull;
+    int k = 0;
 #pragma unroll 2
-    for (int a = part; a < A; a += kApplyParts) {
+    for (int a = part; a < A; a += kApplyParts, ++k) {
       picked |= (accI[a] == i);
+      if (screen_sq<DM>(xf, accXf + a * DM) < thr) flag |= 1ull << k;
+    }
+    while (flag) {
+      const int a = part + kApplyParts * __builtin_ctzll(flag);
+      flag &= flag - 1ull;
       double t = 0.0;
       const double* xa = accX + a * DM;
 #pragma unroll
@@ -398,9 +447,9 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     MMState* __restrict__ stall, int64_t* __restrict__ orderall, int sortcap,
     unsigned long long* __restrict__ prof) {
   // prof (profiling runs only): shader clocks of thread 0 per phase -- set-up,
-  // window checks, pool updates and compactions, re-sorts -- and the number of
-  // steps
-  unsigned long long ph[4] = {0, 0, 0, 0}, steps = 0;
+  // window checks, compactions, re-sorts, pool distance updates -- and the
+  // number of steps
+  unsigned long long ph[5] = {0, 0, 0, 0, 0}, steps = 0;
   unsigned long long ph_t = prof ? __builtin_amdgcn_s_memtime() : 0;
 #define MM_PHASE(k)                                            \
   if (prof) {                                                  \
@@ -434,8 +483,14 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   __shared__ unsigned long long wlow[NW], wlwr[NW];
   __shared__ int stepL;
   __shared__ double wc[kWin * DM], ws2[kWin];
+  __shared__ float wcf[kWin * DM];
+  // this round's accepted points (index, pool slot), written out at the end
+  // of the round: global stores inside the steps would make every barrier
+  // wait for their completion
+  __shared__ int acc_idx[kAccCap], acc_slot[kAccCap];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
+  const double m2d = st->m2d;
   if (count0 >= n) {
     if (tid == 0) st->nacc = 0;
     return;
@@ -602,7 +657,9 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const int nw = nlive < kWin ? nlive : kWin;
     for (int e = tid; e < kWin * DM; e += NT) {
       const int w = e / DM;
-      wc[e] = w < nw ? pc[cs[w] * DM + (e - w * DM)] : 0.0;
+      const double v = w < nw ? pc[cs[w] * DM + (e - w * DM)] : 0.0;
+      wc[e] = v;
+      wcf[e] = (float)v;
     }
     if (tid < kWin) ws2[tid] = tid < nw ? pss[cs[tid]] : 0.0;
     __syncthreads();
@@ -640,18 +697,15 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const int L = stepL;   // >= 1: position 0 is in the pool
     MM_PHASE(1)
     if (tid < L) {
-      const int idx = cidx[tid];
-      acc[nacc + tid] = idx;
-      order[count0 + nacc + tid] = idx;
-#pragma unroll
-      for (int l = 0; l < kMMD; ++l) accx[(nacc + tid) * kMMD + l] = l < DM ? wc[tid * DM + l] : 0.0;
+      acc_idx[nacc + tid] = cidx[tid];
+      acc_slot[nacc + tid] = cs[tid];
     }
     // lower the rest of the pool (position tid) with the accepted points.
     // Lane j of every wave holds accepted point j; the loop broadcasts its
     // coordinates with readlane (scalar operands) instead of LDS reads.
-    double ax[DM];
+    float axf[DM];
 #pragma unroll
-    for (int l = 0; l < DM; ++l) ax[l] = lane < L ? wc[lane * DM + l] : 0.0;
+    for (int l = 0; l < DM; ++l) axf[l] = lane < L ? wcf[lane * DM + l] : 0.0f;
     double key = -3.0;
     int idx = INT_MAX, slot = 0;
     bool lowered = false;
@@ -661,36 +715,51 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
       idx = cidx[tid];
       const double s = pss[slot];
       double x[DM];
+      float xf[DM];
 #pragma unroll
-      for (int l = 0; l < DM; ++l) x[l] = pc[slot * DM + l];
-      // four independent distance chains per iteration (one wave per SIMD:
-      // the FP64 add latency is hidden by instruction-level parallelism)
-      double smin = s;
+      for (int l = 0; l < DM; ++l) {
+        x[l] = pc[slot * DM + l];
+        xf[l] = (float)x[l];
+      }
+      // float32 screen against the L accepted points (lane j of every wave
+      // holds accepted point j: readlane gives scalar operands; four
+      // independent chains per iteration), then the exact canonical
+      // distance for the flagged points only
+      const float thr = screen_threshold(s, m2d);
+      unsigned long long flag = 0ull;
       int j = 0;
       for (; j + 4 <= L; j += 4) {
-        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int l = 0; l < DM; ++l) {
-          const long long b = __double_as_longlong(ax[l]);
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j + u);
-            const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j + u);
-            t[u] = sq_step(t[u], x[l], __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+            const float av = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(axf[l]), j + u));
+            const float df = xf[l] - av;
+            t[u] = fmaf(df, df, t[u]);
           }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) smin = t[u] < smin ? t[u] : smin;
+        for (int u = 0; u < 4; ++u)
+          if (t[u] < thr) flag |= 1ull << (j + u);
       }
       for (; j < L; ++j) {
-        double t = 0.0;
+        float t = 0.0f;
 #pragma unroll
         for (int l = 0; l < DM; ++l) {
-          const long long b = __double_as_longlong(ax[l]);
-          const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
-          const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
-          t = sq_step(t, x[l], __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+          const float av = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(axf[l]), j));
+          const float df = xf[l] - av;
+          t = fmaf(df, df, t);
         }
+        if (t < thr) flag |= 1ull << j;
+      }
+      double smin = s;
+      while (flag) {
+        const int jj = __builtin_ctzll(flag);
+        flag &= flag - 1ull;
+        double t = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) t = sq_step(t, x[l], wc[jj * DM + l]);
         smin = t < smin ? t : smin;
       }
       if (smin < s) {
@@ -702,6 +771,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
           key = -3.0;   // at or below V: leaves the pool
       }
     }
+    MM_PHASE(4)
     const bool live = key >= 0.0;
     // stable compaction of the remaining entries into the other buffer
     const unsigned long long lm = __ballot(live), lw = __ballot(lowered);
@@ -742,16 +812,43 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
       MM_PHASE(3)
     }
   }
+  __syncthreads();
+  for (int k = tid; k < nacc; k += NT) {
+    const int idx = acc_idx[k], sl = acc_slot[k];
+    acc[k] = idx;
+    order[count0 + k] = idx;
+#pragma unroll
+    for (int l = 0; l < kMMD; ++l) accx[k * kMMD + l] = l < DM ? pc[sl * DM + l] : 0.0;
+  }
   if (tid == 0) {
     st->nacc = nacc;
     st->count = count0 + nacc;
     st->rounds += 1;
     if (prof) {
-      for (int k = 0; k < 4; ++k) atomicAdd(prof + k, ph[k]);
-      atomicAdd(prof + 4, steps);
+      for (int k = 0; k < 5; ++k) atomicAdd(prof + k, ph[k]);
+      atomicAdd(prof + 5, steps);
     }
   }
 #undef MM_PHASE
+}
+
+// One workgroup per dataset: m2d = d * max |coordinate|^2 (screen bound).
+__global__ __launch_bounds__(256) void mm_bound_kernel(const MMDesc* __restrict__ desc,
+                                                       const double* __restrict__ XTall,
+                                                       MMState* __restrict__ stall) {
+  __shared__ double red[256];
+  const MMDesc D = desc[blockIdx.x];
+  const double* XT = XTall + D.xoff;
+  const int64_t len = (int64_t)D.n * D.d;
+  double m = 0.0;
+  for (int64_t t = threadIdx.x; t < len; t += 256) m = fmax(m, fabs(XT[t]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) stall[blockIdx.x].m2d = (double)D.d * red[0] * red[0];
 }
 
 __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
@@ -1167,6 +1264,7 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTop + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(mm_bound_kernel, dim3(nd), dim3(256), 0, ctx->stream, dd, dXT, st);
   hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
                      dXT, s2, dist, acc, accx, st, dorder);
   SVH_HIP(hipGetLastError());
@@ -1263,8 +1361,9 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   if (mprof) {
     unsigned long long h[8];
     SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));
-    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu window %llu update %llu "
-                 "resort %llu\n", rounds, h[4], h[0], h[1], h[2], h[3]);
+    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu window %llu "
+                 "distances %llu compaction %llu resort %llu\n", rounds, h[5], h[0], h[1], h[4],
+                 h[2], h[3]);
   }
 }
 
