@@ -267,7 +267,8 @@ class Context:
         Xs = [_finite(X, "X") for X in Xs]
         n = np.array([X.shape[0] for X in Xs], np.int64)
         d = np.array([X.shape[1] for X in Xs], np.int32)
-        first = np.array([int(np.argmin(((X - X.mean(0)) ** 2).sum(1))) for X in Xs], np.int64)
+        from .ordering import maximin_first
+        first = np.array([maximin_first(X) for X in Xs], np.int64)
         Xc = np.concatenate([X.ravel() for X in Xs])
         order = np.empty(int(n.sum()), np.int64)
         nn = np.empty((int(n.sum()), m + 1), np.int64)

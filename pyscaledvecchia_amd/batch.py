@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib
 from .gp import ScaledVecchiaGP
 from .likelihood import _chunk_ids, _chunk_ids_rows
-from .ordering import _nn_groups
+from .ordering import _nn_groups, maximin_first
 
 __all__ = ["fit_batch", "BatchedFit"]
 
@@ -197,8 +197,7 @@ class BatchedFit:
         ms = [preps[i]["m"] for i in which]
 
         def first(k):
-            Xs = preps[which[k]]["Xe"] / lams[k]
-            return int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1)))   # ordering.py:72
+            return maximin_first(preps[which[k]]["Xe"] / lams[k])   # ordering.py:72
         if len(which) > 4:
             # NumPy releases the GIL in these array passes: one host thread per
             # dataset group

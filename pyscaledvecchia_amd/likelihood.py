@@ -11,6 +11,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib
+from .ordering import maximin_first
 
 __all__ = ["vecchia_profile_loglik", "VecchiaStructure", "DeviceRebuild"]
 
@@ -105,7 +106,7 @@ class DeviceRebuild:
         """Reorder at ranges `lam`; returns the maximin order."""
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         Xs = self.Xe / lam
-        first = int(np.argmin(((Xs - Xs.mean(0)) ** 2).sum(1)))   # ordering.py:72
+        first = maximin_first(Xs)   # ordering.py:72
         fresh = getattr(self.ctx, "_rb_owner", None) is not self
         order = self.ctx.vecchia_rebuild(self.Xe if fresh else None, self.ye, self.Ze, lam,
                                          first, self.m, upload=fresh)

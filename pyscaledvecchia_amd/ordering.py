@@ -13,6 +13,22 @@ from . import _lib
 __all__ = ["maximin_order", "find_ordered_nn"]
 
 
+def maximin_first(X: np.ndarray, rows: int = 8192) -> int:
+    """The point closest to the centroid (ordering.py:72):
+    argmin(((X - X.mean(0)) ** 2).sum(1)), first index on ties.  The same
+    NumPy operations run on blocks of rows (each row's sum is independent of
+    the others), so the result is that of the one-pass expression without its
+    full-size temporaries."""
+    mu = X.mean(0)
+    best, at = None, -1
+    for s in range(0, X.shape[0], rows):
+        v = ((X[s:s + rows] - mu) ** 2).sum(1)
+        i = int(np.argmin(v))
+        if best is None or v[i] < best:
+            best, at = v[i], s + i
+    return at
+
+
 def maximin_order(X: np.ndarray, first: int | None = None) -> np.ndarray:
     """Exact maximum-minimum-distance ordering (ordering.py:62-73).
 
@@ -26,7 +42,7 @@ def maximin_order(X: np.ndarray, first: int | None = None) -> np.ndarray:
     if X.shape[0] == 0:
         return np.empty(0, dtype=np.int64)
     if first is None:
-        first = int(np.argmin(((X - X.mean(0)) ** 2).sum(1)))
+        first = maximin_first(X)
     return _lib.context().maximin(X, int(first))
 
 

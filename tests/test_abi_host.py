@@ -114,3 +114,16 @@ def test_estimator_validates_before_device_use():
     with pytest.raises(ValueError):
         ScaledVecchiaGP(nu=None, warm_start="learned").fit(np.random.rand(20, 2),
                                                            np.random.rand(20))
+
+
+@pytest.mark.parametrize("n,d,rows", [(1, 2, 8192), (7, 3, 2), (20000, 8, 8192), (5000, 1, 777)])
+def test_maximin_first_equals_one_pass_expression(n, d, rows):
+    """ordering.py:72 evaluated on row blocks equals the one-pass expression,
+    including the first index among equal distances (duplicated rows)."""
+    from pyscaledvecchia_amd.ordering import maximin_first
+    rng = np.random.default_rng(n + d)
+    X = rng.random((n, d)) / 0.37
+    if n > 10:
+        X[n // 2:] = X[:n - n // 2]          # every distance occurs twice
+    ref = int(np.argmin(((X - X.mean(0)) ** 2).sum(1)))
+    assert maximin_first(X, rows) == ref
