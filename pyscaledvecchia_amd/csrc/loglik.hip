@@ -965,7 +965,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
                                           unsigned long long (&ph)[8]) {
   unsigned long long ph_t = 0;
   if constexpr (PROF) ph_t = __builtin_amdgcn_s_memtime();
-  const int lane = threadIdx.x, h = lane >> 5, r = lane & 31;
+  int lane_o = threadIdx.x;
+  asm volatile("" : "+v"(lane_o));   // per pair: lane-derived offsets are not kept live across pairs
+  const int lane = lane_o, h = lane >> 5, r = lane & 31;
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
   double* hb = lds + h * rows_half_doubles(DM);       // this half's region
@@ -1180,7 +1182,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   // ---- products and records, one block at a time on the whole wave
   const int cyl = grad ? P : 0;
   const int nX = q * q + q;
-#pragma unroll
+#pragma unroll 1
   for (int hh = 0; hh < 2; ++hh) {
     const int64_t bh = b0 + hh;
     if (bh >= a.nblk) break;
