@@ -101,7 +101,7 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 // Batched exact maximin.  Each round:
 //   mm_apply  (grid): apply the points accepted in the previous round to every
 //             unselected point (s2 = min squared distance, dist = sqrt(s2));
-//             each 256-point slice publishes its top-(kSliceTop+1) points by
+//             each 256-point slice publishes its top-(TOP+1) points by
 //             (dist desc, index asc).
 //   mm_select (one workgroup): V = the largest published "next" element over
 //             all slices is an upper bound (in that order) on every point that
@@ -123,7 +123,13 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 // coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
 // the squared distances are bit-identical to the d-term sums.
 constexpr int kSlice = 256;
-constexpr int kSliceTop = 4;          // published candidates per slice
+// published candidates per slice (TOP): 4, or 16 when there are at most
+// kDeepSlices slices (n <= 24,576): small inputs have few slices, and a
+// deeper list per slice gives fewer, fuller rounds (measured, d = 8: 16 is
+// faster up to about 24k points, 4 from about 30k, where the larger pool
+// makes each selection step dearer)
+constexpr int kSliceTopMax = 16;
+constexpr int kDeepSlices = 96;
 constexpr int kPool = 256;            // pool positions = select threads
 constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
@@ -245,17 +251,17 @@ __device__ __forceinline__ void reg_sort(double (&kk)[E], int (&ii)[E]) {
 // points' indices and coordinates (kMMD doubles each, written by mm_select /
 // mm_init) are staged in LDS once per workgroup.  The slice's list is
 // published by register sorts: each wave of part 0 sorts its 64 points, wave
-// 0 sorts the waves' first 32 and writes the first kSliceTop + 1.
+// 0 sorts the waves' first 32 and writes the first TOP + 1.
 constexpr int kApplyParts = 4;
 
-template <int DM>
+template <int DM, int TOP>
 __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     double* __restrict__ s2all, double* __restrict__ distall, const int* __restrict__ accall,
     const double* __restrict__ accxall, const MMState* __restrict__ stall,
     double* __restrict__ cand_keyall, int* __restrict__ cand_idxall) {
   constexpr int PW = kSlice / kWave;   // waves of part 0
-  static_assert(kSliceTop + 1 <= 32, "published list exceeds half a wave");
+  static_assert(TOP + 1 <= 32, "published list exceeds half a wave");
   static_assert(kAccCap <= 64 * kApplyParts, "screen mask holds 64 points per part");
   __shared__ double accX[kAccCap * DM];
   __shared__ float accXf[kAccCap * DM];
@@ -273,8 +279,8 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
   const int* acc = accall + (int64_t)blockIdx.y * kAccCap;
   const MMState* st = stall + blockIdx.y;
   if (st->count >= n) return;   // order complete: nothing left to select
-  double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
-  int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
+  double* cand_key = cand_keyall + (int64_t)D.slice0 * (TOP + 1);
+  int* cand_idx = cand_idxall + (int64_t)D.slice0 * (TOP + 1);
   const double* accx = accxall + (int64_t)blockIdx.y * kAccCap * kMMD;
   const int A = st->nacc;
   // part is the same on a whole wave (kSlice is a multiple of the wave size)
@@ -357,13 +363,13 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
   }
   __syncthreads();
   if (wid == 0) {
-    // the slice's top-(kSliceTop+1) from the waves' first 32
+    // the slice's top-(TOP+1) from the waves' first 32
     double kk[2] = {tk[lane], tk[kWave + lane]};
     int ii[2] = {ti[lane], ti[kWave + lane]};
     reg_sort<MaxFirst, 2>(kk, ii);
-    if (lane <= kSliceTop) {
-      cand_key[blockIdx.x * (kSliceTop + 1) + lane] = kk[0];
-      cand_idx[blockIdx.x * (kSliceTop + 1) + lane] = ii[0];
+    if (lane <= TOP) {
+      cand_key[blockIdx.x * (TOP + 1) + lane] = kk[0];
+      cand_idx[blockIdx.x * (TOP + 1) + lane] = ii[0];
     }
   }
 }
@@ -439,7 +445,7 @@ __device__ __forceinline__ void pool_sort(double& k, int& i, int& s, double* bk,
   pool_sort_size<NT, 2>(k, i, s, bk, bi, bs);
 }
 
-template <int DM>
+template <int DM, int TOP>
 __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
     const double* __restrict__ s2all, const double* __restrict__ cand_keyall,
@@ -463,8 +469,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   const int n = D.n, d = D.d, nslice = D.nslice;
   const double* XT = XTall + D.xoff;
   const double* s2g = s2all + D.soff;
-  const double* cand_key = cand_keyall + (int64_t)D.slice0 * (kSliceTop + 1);
-  const int* cand_idx = cand_idxall + (int64_t)D.slice0 * (kSliceTop + 1);
+  const double* cand_key = cand_keyall + (int64_t)D.slice0 * (TOP + 1);
+  const int* cand_idx = cand_idxall + (int64_t)D.slice0 * (TOP + 1);
   int* acc = accall + (int64_t)blockIdx.x * kAccCap;
   double* accx = accxall + (int64_t)blockIdx.x * kAccCap * kMMD;
   MMState* st = stall + blockIdx.x;
@@ -495,14 +501,14 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     if (tid == 0) st->nacc = 0;
     return;
   }
-  // bound V: the largest "next" element (index kSliceTop) over all slices;
+  // bound V: the largest "next" element (index TOP) over all slices;
   // every point outside the slices' published lists is at or below it
-  const int T = kSliceTop;
+  const int T = TOP;
   double Vk = -3.0;
   int Vi = INT_MAX;
   for (int s = tid; s < nslice; s += NT) {
-    const double k = cand_key[s * (kSliceTop + 1) + T];
-    const int i = cand_idx[s * (kSliceTop + 1) + T];
+    const double k = cand_key[s * (TOP + 1) + T];
+    const int i = cand_idx[s * (TOP + 1) + T];
     if (mm_greater(k, i, Vk, Vi)) {
       Vk = k;
       Vi = i;
@@ -520,8 +526,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     bool keep = false;
     if (c < ncand) {
       const int s = c / T, r = c - s * T;
-      k = cand_key[s * (kSliceTop + 1) + r];
-      i = cand_idx[s * (kSliceTop + 1) + r];
+      k = cand_key[s * (TOP + 1) + r];
+      i = cand_idx[s * (TOP + 1) + r];
       keep = k >= 0.0 && mm_greater(k, i, Vk, Vi);
     }
     const unsigned long long m = __ballot(keep);
@@ -552,8 +558,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     double Gk = -3.0;
     int Gi = INT_MAX;
     for (int s = tid; s < nslice; s += NT) {
-      const double k = cand_key[s * (kSliceTop + 1)];
-      const int i = cand_idx[s * (kSliceTop + 1)];
+      const double k = cand_key[s * (TOP + 1)];
+      const int i = cand_idx[s * (TOP + 1)];
       if (mm_greater(k, i, Gk, Gi)) {
         Gk = k;
         Gi = i;
@@ -565,8 +571,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     int Si = INT_MAX;
     for (int s = tid; s < nslice; s += NT) {
       const int e = (s == gs) ? 1 : 0;   // the next entry of G's own list
-      const double k = cand_key[s * (kSliceTop + 1) + e];
-      const int i = cand_idx[s * (kSliceTop + 1) + e];
+      const double k = cand_key[s * (TOP + 1) + e];
+      const int i = cand_idx[s * (TOP + 1) + e];
       if (mm_greater(k, i, Sk, Si)) {
         Sk = k;
         Si = i;
@@ -1242,6 +1248,50 @@ void launch_gather_soa(svh_ctx* ctx, const double* dXT, const int64_t* dorder, i
 
 // Maximin orders of several problems at once: rounds of (apply, select)
 // launches over all unfinished problems until every order is complete.
+// One maximin round (apply, then select) for coordinate registers DM and
+// published depth TOP.
+struct MMRound {
+  const MMDesc* dd;
+  const double* XT;
+  double *s2, *dist;
+  int* acc;
+  double* accx;
+  MMState* st;
+  double* ck;
+  int* ci;
+  int64_t* order;
+  int sortcap;
+  size_t select_lds;
+  unsigned long long* prof;
+  dim3 ga;
+  int nd;
+};
+
+template <int DM, int TOP>
+void mm_round(svh_ctx* ctx, const MMRound& R) {
+  ctx->timers.begin(ctx->stream, "maximin_apply");
+  hipLaunchKernelGGL((mm_apply_kernel<DM, TOP>), R.ga, dim3(kSlice * kApplyParts), 0, ctx->stream,
+                     R.dd, R.XT, R.s2, R.dist, R.acc, R.accx, R.st, R.ck, R.ci);
+  ctx->timers.end(ctx->stream);
+  ctx->timers.begin(ctx->stream, "maximin_select");
+  hipLaunchKernelGGL((mm_select_kernel<DM, TOP>), dim3(R.nd), dim3(kPool), R.select_lds,
+                     ctx->stream, R.dd, R.XT, R.s2, R.ck, R.ci, R.acc, R.accx, R.st, R.order,
+                     R.sortcap, R.prof);
+  ctx->timers.end(ctx->stream);
+}
+
+template <int TOP>
+void mm_round_dm(svh_ctx* ctx, int dm, const MMRound& R) {
+  if (dm == 4)
+    mm_round<4, TOP>(ctx, R);
+  else if (dm == 8)
+    mm_round<8, TOP>(ctx, R);
+  else if (dm == 10)
+    mm_round<10, TOP>(ctx, R);
+  else
+    mm_round<12, TOP>(ctx, R);
+}
+
 void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMDesc>& hd,
                           int64_t* dorder) {
   const int nd = (int)hd.size();
@@ -1260,8 +1310,8 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int* acc = ctx->arena.get_t<int>("mm.acc", (size_t)nd * kAccCap);
   double* accx = ctx->arena.get_t<double>("mm.accx", (size_t)nd * kAccCap * kMMD);
   MMState* st = ctx->arena.get_t<MMState>("mm.st", nd);
-  double* ck = ctx->arena.get_t<double>("mm.ck", (size_t)nslice_tot * (kSliceTop + 1));
-  int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTop + 1));
+  double* ck = ctx->arena.get_t<double>("mm.ck", (size_t)nslice_tot * (kSliceTopMax + 1));
+  int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTopMax + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(mm_bound_kernel, dim3(nd), dim3(256), 0, ctx->stream, dd, dXT, st);
@@ -1272,8 +1322,15 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   SVH_REQUIRE(dmax <= kMMD, "maximin: d above 12 not supported");
   // coordinate registers per point: d rounded up to 4, 8, 10 or 12 (zero padded)
   const int dm = dmax <= 4 ? 4 : (dmax <= 8 ? 8 : (dmax <= 10 ? 10 : 12));
+  // published list depth: deep lists for few slices (SVH_MM_TOP = 4 / 16
+  // overrides, for tests)
+  int top = maxslice <= kDeepSlices ? 16 : 4;
+  if (const char* e = std::getenv("SVH_MM_TOP")) {
+    const int v = std::atoi(e);
+    if (v == 4 || v == 16) top = v;
+  }
   int sortcap = 1;
-  while (sortcap < maxslice * kSliceTop && sortcap < kSortCap) sortcap <<= 1;
+  while (sortcap < maxslice * top && sortcap < kSortCap) sortcap <<= 1;
   if (sortcap < kPool) sortcap = kPool;
   // SVH_MM_SORTCAP (tests only): a smaller survivor capacity, so that the
   // overflow path of mm_select runs at small n
@@ -1299,36 +1356,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   }
   for (;;) {
     for (int r = 0; r < batch; ++r) {
-      ctx->timers.begin(ctx->stream, "maximin_apply");
-      const dim3 ga(maxslice, nd);
-      const dim3 ba(kSlice * kApplyParts);
-      if (dm == 4)
-        hipLaunchKernelGGL(mm_apply_kernel<4>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, accx, st, ck, ci);
-      else if (dm == 8)
-        hipLaunchKernelGGL(mm_apply_kernel<8>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, accx, st, ck, ci);
-      else if (dm == 10)
-        hipLaunchKernelGGL(mm_apply_kernel<10>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, accx, st, ck, ci);
+      const MMRound R{dd, dXT, s2, dist, acc, accx, st, ck, ci, dorder, sortcap, select_lds, mprof,
+                      dim3(maxslice, nd), nd};
+      if (top == 16)
+        mm_round_dm<16>(ctx, dm, R);
       else
-        hipLaunchKernelGGL(mm_apply_kernel<12>, ga, ba, 0, ctx->stream, dd, dXT, s2, dist,
-                           acc, accx, st, ck, ci);
-      ctx->timers.end(ctx->stream);
-      ctx->timers.begin(ctx->stream, "maximin_select");
-      if (dm == 4)
-        hipLaunchKernelGGL(mm_select_kernel<4>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
-      else if (dm == 8)
-        hipLaunchKernelGGL(mm_select_kernel<8>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
-      else if (dm == 10)
-        hipLaunchKernelGGL(mm_select_kernel<10>, dim3(nd), dim3(kPool), select_lds, ctx->stream,
-                           dd, dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
-      else
-        hipLaunchKernelGGL(mm_select_kernel<12>, dim3(nd), dim3(kPool), select_lds, ctx->stream, dd,
-                           dXT, s2, ck, ci, acc, accx, st, dorder, sortcap, mprof);
-      ctx->timers.end(ctx->stream);
+        mm_round_dm<4>(ctx, dm, R);
     }
     SVH_HIP(hipGetLastError());
     SVH_HIP(hipMemcpyAsync(hst, st, nd * sizeof(MMState), hipMemcpyDeviceToHost, ctx->stream));

@@ -83,6 +83,22 @@ def test_maximin_survivor_overflow_stays_exact(gpu, monkeypatch):
     assert gpu.timing_read("maximin_fallbacks")[1] > 0
 
 
+@pytest.mark.parametrize("top", ["4", "16"])
+def test_maximin_both_list_depths_exact(gpu, monkeypatch, top):
+    """Slices publish 4 or 16 candidates (16 by default up to 64 slices);
+    both depths give the serial order, on random points and on a grid with
+    exact ties."""
+    from oracle import c_oracle as C
+    monkeypatch.setenv("SVH_MM_TOP", top)
+    X = np.random.default_rng(31).random((20000, 5)) / 0.3
+    g = np.arange(12.0)
+    G = np.array([[a, b, c] for a in g for b in g for c in g])
+    for Y in (X, G):
+        first = O.maximin_first(Y)
+        ref = C.maximin_order(Y, first) if C.available() else O.maximin_order(Y, first)
+        np.testing.assert_array_equal(gpu.maximin(Y, first), ref)
+
+
 @pytest.mark.parametrize("n,d,m,start,seed", [(120, 3, 8, 0, 2), (2000, 4, 30, 0, 7),
                                               (600, 8, 30, 400, 8), (60, 2, 6, 40, 4),
                                               (300, 1, 5, 0, 9), (200, 5, 140, 0, 10),
