@@ -455,7 +455,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   // prof (profiling runs only): shader clocks of thread 0 per phase -- set-up,
   // window checks, compactions, re-sorts, pool distance updates -- and the
   // number of steps
-  unsigned long long ph[5] = {0, 0, 0, 0, 0}, steps = 0;
+  unsigned long long ph[7] = {0, 0, 0, 0, 0, 0, 0}, steps = 0;
   unsigned long long ph_t = prof ? __builtin_amdgcn_s_memtime() : 0;
 #define MM_PHASE(k)                                            \
   if (prof) {                                                  \
@@ -515,6 +515,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     }
   }
   block_argmax<NT>(Vk, Vi, rk, ri);
+  MM_PHASE(5)
   // survivors: first-T candidates strictly above V, compacted into LDS (the
   // first sortcap of them; cnt counts all)
   int cnt = 0;
@@ -630,6 +631,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     if (tid == 0) st->overflow_sorts += 1;
     __syncthreads();
   }
+  MM_PHASE(6)
   // pool: slot t holds survivor t
   if (tid < nsurv) {
     const int pi = si[tid];
@@ -831,8 +833,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     st->count = count0 + nacc;
     st->rounds += 1;
     if (prof) {
-      for (int k = 0; k < 5; ++k) atomicAdd(prof + k, ph[k]);
-      atomicAdd(prof + 5, steps);
+      for (int k = 0; k < 7; ++k) atomicAdd(prof + k, ph[k]);
+      atomicAdd(prof + 7, steps);
     }
   }
 #undef MM_PHASE
@@ -1394,9 +1396,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   if (mprof) {
     unsigned long long h[8];
     SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));
-    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup %llu window %llu "
-                 "distances %llu compaction %llu resort %llu\n", rounds, h[5], h[0], h[1], h[4],
-                 h[2], h[3]);
+    int sorts = 0;
+    for (int i = 0; i < nd; ++i) sorts += hst[i].overflow_sorts;
+    std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup: bound %llu survivors "
+                 "%llu pool %llu; window %llu distances %llu compaction %llu resort %llu "
+                 "survivor-sorts %d\n", rounds, h[7], h[5], h[6], h[0], h[1], h[4], h[2], h[3],
+                 sorts);
   }
 }
 
