@@ -143,7 +143,9 @@ struct MMState {
   int overflow_sorts;
   int fallbacks;   // rounds that kept only the largest candidate (survivors > sortcap)
   int pad;
-  double m2d;      // d * max |coordinate|^2 of the dataset (float32 screen bound)
+  // bits of the dataset's largest |coordinate| (non-negative doubles order as
+  // their bit patterns): the float32 screen uses m2d = d * max|coordinate|^2
+  unsigned long long mabs;
 };
 
 // One ordering problem of a batch (several datasets share each launch).
@@ -310,7 +312,8 @@ __global__ __launch_bounds__(kSlice * kApplyParts) void mm_apply_kernel(
     // float32 screen over this part's accepted points (bit k: point
     // part + kApplyParts * k, at most 64 of them), then the exact canonical
     // distance for the flagged ones only
-    const float thr = screen_threshold(s2[i], st->m2d);
+    const double M = __longlong_as_double((long long)st->mabs);
+    const float thr = screen_threshold(s2[i], (double)d * M * M);
     unsigned long long flag = 0ull;
     int k = 0;
 #pragma unroll 2
@@ -496,7 +499,8 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   __shared__ int acc_idx[kAccCap], acc_slot[kAccCap];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
-  const double m2d = st->m2d;
+  const double Mabs = __longlong_as_double((long long)st->mabs);
+  const double m2d = (double)d * Mabs * Mabs;
   if (count0 >= n) {
     if (tid == 0) st->nacc = 0;
     return;
@@ -840,23 +844,10 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
 #undef MM_PHASE
 }
 
-// One workgroup per dataset: m2d = d * max |coordinate|^2 (screen bound).
-__global__ __launch_bounds__(256) void mm_bound_kernel(const MMDesc* __restrict__ desc,
-                                                       const double* __restrict__ XTall,
-                                                       MMState* __restrict__ stall) {
-  __shared__ double red[256];
-  const MMDesc D = desc[blockIdx.x];
-  const double* XT = XTall + D.xoff;
-  const int64_t len = (int64_t)D.n * D.d;
-  double m = 0.0;
-  for (int64_t t = threadIdx.x; t < len; t += 256) m = fmax(m, fabs(XT[t]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) stall[blockIdx.x].m2d = (double)D.d * red[0] * red[0];
+// Clears the per-dataset coordinate bound before mm_init accumulates it.
+__global__ void mm_zero_kernel(MMState* __restrict__ stall, int nd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nd) stall[i].mabs = 0ull;
 }
 
 __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
@@ -864,10 +855,16 @@ __global__ void mm_init_kernel(const MMDesc* __restrict__ desc, const double* __
                                MMState* stall, int64_t* orderall) {
   const MMDesc D = desc[blockIdx.y];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 0.0;
   if (i < D.n) {
     s2all[D.soff + i] = __longlong_as_double(0x7ff0000000000000ll);  // +inf
     distall[D.soff + i] = __longlong_as_double(0x7ff0000000000000ll);
+    for (int l = 0; l < D.d; ++l) m = fmax(m, fabs(XTall[D.xoff + (int64_t)l * D.n + i]));
   }
+  // the dataset's largest |coordinate|: wave maximum, one atomic per wave
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(&stall[blockIdx.y].mabs, (unsigned long long)__double_as_longlong(m));
   if (i == 0) {
     accall[(int64_t)blockIdx.y * kAccCap] = D.first;
     for (int l = 0; l < kMMD; ++l)
@@ -1316,7 +1313,7 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTopMax + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(mm_bound_kernel, dim3(nd), dim3(256), 0, ctx->stream, dd, dXT, st);
+  hipLaunchKernelGGL(mm_zero_kernel, dim3((nd + 255) / 256), dim3(256), 0, ctx->stream, st, nd);
   hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
                      dXT, s2, dist, acc, accx, st, dorder);
   SVH_HIP(hipGetLastError());
