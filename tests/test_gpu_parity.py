@@ -85,7 +85,7 @@ def test_maximin_survivor_overflow_stays_exact(gpu, monkeypatch):
 
 @pytest.mark.parametrize("top", ["4", "16"])
 def test_maximin_both_list_depths_exact(gpu, monkeypatch, top):
-    """Slices publish 4 or 16 candidates (16 by default up to 64 slices);
+    """Slices publish 4 or 16 candidates (16 by default up to 96 slices);
     both depths give the serial order, on random points and on a grid with
     exact ties."""
     from oracle import c_oracle as C
