@@ -886,22 +886,10 @@ __device__ __forceinline__ double half_sum(double v) {
   return a0 + a1;
 }
 
-// v of lane SRC of this lane's half (SRC in 0..31), for both halves at once;
-// all lanes active.  row_newbcast gives every 16-lane row its lane SRC % 16,
-// the row pair swap then hands row SRC / 16 of each half to both of its rows.
-template <int SRC>
-__device__ __forceinline__ double half_bcast(double v) {
-  static_assert(SRC >= 0 && SRC < 32, "source lane of a half");
-  const double t = __builtin_amdgcn_update_dpp(0.0, v, 0x150 + (SRC & 15), 0xF, 0xF, false);
-  const long long b = __double_as_longlong(t);
-  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
-  const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-  constexpr int o = SRC < 16 ? 0 : 1;
-  return __longlong_as_double(((long long)sh[o] << 32) | sl[o]);
-}
-
-// float v of lane SRC of this lane's half (the 32-bit form of half_bcast).
+// float v of lane SRC of this lane's half (SRC in 0..31), for both halves at
+// once; all lanes active.  row_newbcast gives every 16-lane row its lane
+// SRC % 16, the row pair swap then hands row SRC / 16 of each half to both of
+// its rows.
 template <int SRC>
 __device__ __forceinline__ float half_bcast_f(float v) {
   static_assert(SRC >= 0 && SRC < 32, "source lane of a half");
@@ -912,32 +900,190 @@ __device__ __forceinline__ float half_bcast_f(float v) {
   return __int_as_float((int)sw[o]);
 }
 
-// LDS doubles per wave of the rows kernel: per half [cb 32 | bb 16 | ab 32 |
-// (unused) 16 | xs 32*dm] behind one base, then misc (16) and Mt (24 x 33)
-__host__ __device__ inline int rows_half_doubles(int dm) { return 96 + 32 * dm; }
+// LDS doubles per wave of the rows kernel: per half [ab 32 | xs 32*dm], then
+// the packed lower triangles of g(r) of both halves (2 x 496; the product
+// operands Mt, 24 x 33, reuse that space once g is consumed), then misc (16).
+constexpr int kGTri = 496;   // 32 * 31 / 2 strictly-lower entries
+__host__ __device__ inline int rows_half_doubles(int dm) { return 32 + 32 * dm; }
 __host__ __device__ inline int rows_lds_doubles(int dm) {
-  return 2 * rows_half_doubles(dm) + 16 + 24 * 33;
+  return 2 * rows_half_doubles(dm) + 2 * kGTri + 16;
+}
+// Entry (i, j), i > j, of a strictly-lower 32 x 32 triangle packed by
+// columns: column j holds rows j+1..31 contiguously, so g_base(j) + i.
+__host__ __device__ constexpr int g_base(int j) { return 30 * j - (j * (j - 1)) / 2 - 1; }
+
+// ---- fused lane broadcasts
+// acc -= src[lane N of this lane's 16-lane row] * m as one v_fmac_f64 with a
+// DPP row_newbcast source operand (the only DPP control that 64-bit VALU
+// operations take).  The compiler neither forms this instruction nor tracks
+// the DPP read-after-VALU-write hazard through inline assembly, so each
+// statement starts with the two wait states that hazard needs.  Callers run
+// them with EXEC unchanged (no divergent branches around them).
+#define SVH_BF(acc, src, m, n) \
+  "v_fmac_f64_dpp %" #acc ", %" #src ", -%" #m " row_newbcast:%" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1"
+template <int N>
+__device__ __forceinline__ void bf1(double& a0, double s0, double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 1, 2, 3) : "+v"(a0) : "v"(s0), "v"(m), "i"(N));
+}
+// four accumulators, one source, four source lanes
+template <int N0, int N1, int N2, int N3>
+__device__ __forceinline__ void bf4_lanes(double& a0, double& a1, double& a2, double& a3,
+                                          double s, double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 4, 5, 6) "\n\t" SVH_BF(1, 4, 5, 7) "\n\t" SVH_BF(
+                   2, 4, 5, 8) "\n\t" SVH_BF(3, 4, 5, 9)
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+               : "v"(s), "v"(m), "i"(N0), "i"(N1), "i"(N2), "i"(N3));
+}
+// four accumulators, each its own source, one source lane
+template <int N>
+__device__ __forceinline__ void bf4_self(double& a0, double& a1, double& a2, double& a3,
+                                         double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 0, 4, 5) "\n\t" SVH_BF(1, 1, 4, 5) "\n\t" SVH_BF(
+                   2, 2, 4, 5) "\n\t" SVH_BF(3, 3, 4, 5)
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+               : "v"(m), "i"(N));
+}
+// four accumulators, four separate sources, one source lane
+template <int N>
+__device__ __forceinline__ void bf4_src(double& a0, double& a1, double& a2, double& a3,
+                                        double s0, double s1, double s2, double s3, double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 4, 8, 9) "\n\t" SVH_BF(1, 5, 8, 9) "\n\t" SVH_BF(
+                   2, 6, 8, 9) "\n\t" SVH_BF(3, 7, 8, 9)
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+               : "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(m), "i"(N));
+}
+#undef SVH_BF
+
+// A copy of a lane index that the compiler cannot see through: each phase
+// compares its own copy, so lane masks are formed where they are used rather
+// than shared (and kept live) across phases.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
 }
 
-// Step K of W = L'^-1 (dS a) and the steps after it: row K's right-hand
-// sides (lane K of each half) broadcast by lane moves, eliminated below the
-// diagonal.
-template <int K>
-__device__ __forceinline__ void w_sweep(double (&R)[kRC], const double (&S)[32], int r) {
-  const double l = (r > K) ? S[K] : 0.0;
-  // four columns at a time (bounded register pressure)
-#pragma unroll
-  for (int c0 = 0; c0 < kRC; c0 += 4) {
-    double wk[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) wk[u] = half_bcast<K>(R[c0 + u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) R[c0 + u] = fma(-l, wk[u], R[c0 + u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(R[c0 + u]));
-    __builtin_amdgcn_sched_barrier(0);
+// v of lane N of this lane's 16-lane row (compiler-managed v_mov_b64 DPP)
+template <int N>
+__device__ __forceinline__ double row_bc(double v) {
+  return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + N, 0xF, 0xF, true);
+}
+
+// The first 16-lane row's v in both 16-lane rows of each half
+// (v_permlane16_swap: output 0 carries the even rows to both rows of a pair).
+__device__ __forceinline__ double from_lo_row(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto sh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __longlong_as_double(((long long)sh[0] << 32) | sl[0]);
+}
+
+// Trailing update of step k for columns C..E-1: S[c] -= src(lane c - LOFF) * m
+template <int C, int E, int LOFF>
+__device__ __forceinline__ void fac_trail(double (&S)[32], double src, double m) {
+  if constexpr (E - C >= 4) {
+    bf4_lanes<C - LOFF, C + 1 - LOFF, C + 2 - LOFF, C + 3 - LOFF>(S[C], S[C + 1], S[C + 2],
+                                                                  S[C + 3], src, m);
+    fac_trail<C + 4, E, LOFF>(S, src, m);
+  } else if constexpr (E - C >= 1) {
+    bf1<C - LOFF>(S[C], src, m);
+    fac_trail<C + 1, E, LOFF>(S, src, m);
   }
-  if constexpr (K + 1 < 31) w_sweep<K + 1>(R, S, r);
+}
+
+// Twelve right-hand-side columns: V[c] -= V[c](lane N) * m
+template <int N>
+__device__ __forceinline__ void rhs_self(double (&V)[12], double m) {
+  bf4_self<N>(V[0], V[1], V[2], V[3], m);
+  bf4_self<N>(V[4], V[5], V[6], V[7], m);
+  bf4_self<N>(V[8], V[9], V[10], V[11], m);
+}
+
+// Rows 16..31 of each half take their updates from the finished rows 0..15:
+// V[c] -= sum_{j<16} l_rj V_j[c] in order j = 0..15 (l_rj = S[j] there).
+template <int J>
+__device__ __forceinline__ void rhs_from_lo(double (&V)[12], const double (&W)[12],
+                                            double (&S)[32], bool hi_row) {
+  if constexpr (J < 16) {
+    asm volatile("" : "+v"(S[J]));
+    const double m = hi_row ? S[J] : 0.0;
+    bf4_src<J>(V[0], V[1], V[2], V[3], W[0], W[1], W[2], W[3], m);
+    bf4_src<J>(V[4], V[5], V[6], V[7], W[4], W[5], W[6], W[7], m);
+    bf4_src<J>(V[8], V[9], V[10], V[11], W[8], W[9], W[10], W[11], m);
+    rhs_from_lo<J + 1>(V, W, S, hi_row);
+  }
+}
+__device__ __forceinline__ void rhs_lo_to_hi(double (&V)[12], double (&S)[32], int r) {
+  double W[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) W[c] = from_lo_row(V[c]);
+  rhs_from_lo<0>(V, W, S, r >= 16);
+}
+
+// Step k of the unit LDL^T factorisation of the rows kernel, with the rows of
+// L'^-1 [y Z] (bv) carried along.  Lane r holds row r; rows 0..15 of a block
+// sit in the first 16-lane row of its half, rows 16..31 in the second.
+// Column k (lane c's S[k] = S_ck) reaches the other lanes by DPP row
+// broadcasts: for c < 16 from the first row's values copied to both rows
+// (one v_permlane16_swap pair per step, k < 16 only), for c >= 16 from the
+// second row directly.  Right-hand sides: rows 0..15 update at steps 0..14;
+// rows 16..31 take steps 0..15 together after step 15 (rhs_lo_to_hi), then
+// steps 16..30 -- the same operations in the same order per row as a plain
+// right-looking sweep.
+template <int k>
+__device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[12], double& pr,
+                                         bool& fail, int r, int K) {
+  double src;
+  if constexpr (k < 16)
+    src = from_lo_row(S[k]);
+  else
+    src = S[k];
+  const double pk = row_bc<k & 15>(src);   // S_kk (exact in the second row for every k)
+  fail |= (k < K) && !(pk > 0.0);
+  const double ik = rcp_nr(pk);
+  if (r == k) pr = pk;
+  const double lrk = S[k] * ik;
+  if constexpr (k < 15) fac_trail<k + 1, 16, 0>(S, src, lrk);
+  fac_trail<(k + 1 > 16 ? k + 1 : 16), 32, 16>(S, S[k], lrk);
+  S[k] = (r > k) ? lrk : S[k];
+  asm volatile("" : "+v"(S[k]));   // the select here, not deferred with both values live
+  if constexpr (k < 16) {
+    rhs_self<k>(bv, (r > k && r < 16) ? lrk : 0.0);
+    if constexpr (k == 15) rhs_lo_to_hi(bv, S, r);
+  } else {
+    rhs_self<k - 16>(bv, (r > k) ? lrk : 0.0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (k + 1 < 32) fac_step<k + 1>(S, bv, pr, fail, r, K);
+}
+
+// W = L'^-1 R for the rows kernel's twelve columns (unit lower L' in S):
+// rows 0..15 with steps 0..14, rows 16..31 with steps 0..15 from the finished
+// first rows, then steps 16..30.
+// The multipliers are formed at their step (S[K] passed through an empty asm
+// there): computed ahead, all 31 would be live at once.
+template <int K>
+__device__ __forceinline__ void wsw_lo(double (&R)[12], double (&S)[32], int r) {
+  if constexpr (K < 15) {
+    asm volatile("" : "+v"(S[K]));
+    rhs_self<K>(R, (r > K && r < 16) ? S[K] : 0.0);
+    __builtin_amdgcn_sched_barrier(0);
+    wsw_lo<K + 1>(R, S, r);
+  }
+}
+template <int K>
+__device__ __forceinline__ void wsw_hi(double (&R)[12], double (&S)[32], int r) {
+  if constexpr (K < 31) {
+    asm volatile("" : "+v"(S[K]));
+    rhs_self<K - 16>(R, (r > K) ? S[K] : 0.0);
+    __builtin_amdgcn_sched_barrier(0);
+    wsw_hi<K + 1>(R, S, r);
+  }
+}
+__device__ __forceinline__ void w_sweep_dpp(double (&R)[12], double (&S)[32], int r) {
+  wsw_lo<0>(R, S, r);
+  rhs_lo_to_hi(R, S, r);
+  wsw_hi<16>(R, S, r);
 }
 
 // w_sweep in float32 (mixed mode): the unit factor's entries rounded to float,
@@ -971,12 +1117,12 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   const int d = a.d, q = a.q, P = a.P;
   const bool grad = a.need_grad != 0;
   double* hb = lds + h * rows_half_doubles(DM);       // this half's region
-  double* cb = hb;                                    // column broadcast
-  double* bb = hb + 32;                               // right-hand-side row broadcast
-  double* ab = hb + 48;                               // a
-  double* xs = hb + 96;                               // coordinates of this half's block
-  double* misc = lds + 2 * rows_half_doubles(DM);     // per half: [8h] fail, [8h+1..2] pivot
-  double* Mt = misc + 16;                             // 24 x 33 product operands
+  double* ab = hb;                                    // a
+  double* xs = hb + 32;                               // coordinates of this half's block
+  double* gt = lds + 2 * rows_half_doubles(DM);       // packed lower g(r), both halves
+  double* gh = gt + h * kGTri;                        // this half's g
+  double* Mt = gt;                                    // 24 x 33 product operands (after g)
+  double* misc = gt + 2 * kGTri;                      // per half: [8h] fail, [8h+1..2] pivot
   const int64_t b = b0 + h;   // first block of the pair: b0
   const bool bvalid = b < a.nblk;
   const int K = bvalid ? a.kcnt[b] + 1 : 1;
@@ -1018,11 +1164,14 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   }
   wave_sync();
   SVH_PHASE(0)
-  // ---- Sigma row r, lower part (upper part zero)
+  // ---- Sigma row r, lower part (upper part zero); g(r) of the lower pairs
+  // to the packed triangle (zero for padding rows) for the dS a phase
   double S[32];
+  {
+  const int r = opaque(lane_o & 31);
 #pragma unroll
   for (int c = 0; c < 32; ++c) {
-    double v = 0.0;
+    double v = 0.0, gs = 0.0;
     if (c == r) {
       v = act ? var * (1.0 + nug) + jit : 1.0;
     } else if (c < r && act) {
@@ -1038,6 +1187,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         float M, g;
         matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
         v = var * (double)M;
+        gs = (double)g;
       } else {
         double s2 = 0.0;
 #pragma unroll
@@ -1048,44 +1198,21 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         double M, g;
         matern_t<NU>(sqrt(s2), M, g, a.nu_code);
         v = var * M;
+        gs = g;
       }
     }
+    if (c < r) gh[g_base(c) + r] = gs;
     S[c] = v;
     __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
   }
+  }
   SVH_PHASE(1)
-  // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly
+  // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly,
+  // cross-lane values by DPP row broadcasts (fac_step)
   double pr = 1.0;
   bool fail = false;
-#pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    cb[r] = S[k];
-    if (r == k) {
-#pragma unroll
-      for (int c = 0; c < 1 + kRQ; ++c) bb[c] = bv[c];
-    }
-    wave_sync();
-    const double pk = cb[k];
-    fail |= (k < K) && !(pk > 0.0);
-    const double ik = rcp_nr(pk);
-    if (r == k) pr = pk;
-    const double lrk = S[k] * ik;
-#pragma unroll
-    for (int c = k + 1; c < 32; ++c) S[c] = fma(-lrk, cb[c], S[c]);
-    if (r > k) {
-      S[k] = lrk;
-#pragma unroll
-      for (int c = 0; c < 1 + kRQ; ++c) bv[c] = fma(-lrk, bb[c], bv[c]);
-    }
-    // keep each step's updates in place: left alone, the compiler defers them
-    // to the step that reads each entry and holds every broadcast column live
-#pragma unroll
-    for (int c = k + 1; c < 32; ++c) asm volatile("" : "+v"(S[c]));
-#pragma unroll
-    for (int c = 0; c < 1 + kRQ; ++c) asm volatile("" : "+v"(bv[c]));
-    wave_sync();
-    __builtin_amdgcn_sched_barrier(0);
-  }
+  asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
+  fac_step<0>(S, bv, pr, fail, opaque(r), K);
   const double dinv = rsqrt_nr(pr);   // 1 / L_rr
 #pragma unroll
   for (int c = 0; c < 1 + kRQ; ++c) bv[c] *= dinv;   // rows of L^-1 [y Z]
@@ -1094,7 +1221,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     misc[8 * h + 1] = pr;
     misc[8 * h + 2] = dinv;
   }
-  if (r == 0) misc[8 * h] = fail ? 1.0 : 0.0;
+  if (r == 31) misc[8 * h] = fail ? 1.0 : 0.0;   // the second 16-lane row saw every pivot
   wave_sync();
   const double pK = misc[8 * h + 1], dK = misc[8 * h + 2];
   SVH_PHASE(2)
@@ -1103,18 +1230,23 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   for (int c = 0; c < kRC; ++c) R[c] = 0.0;
   if (grad) {
     // ---- t = L'^-T e_{K-1}, a = t / L_KK
-    double t = (r == K - 1) ? 1.0 : 0.0;
+    const int ra = opaque(r);
+    double t = (ra == K - 1) ? 1.0 : 0.0;
 #pragma unroll
     for (int c = 30; c >= 0; --c) {
-      const double s = half_sum((r > c) ? S[c] * t : 0.0);
-      if (r == c) t -= s;
+      const double s = half_sum((ra > c) ? S[c] * t : 0.0);
+      if (ra == c) t -= s;
       __builtin_amdgcn_sched_barrier(0);
     }
     const double av = t * dK;
     ab[r] = av;
     wave_sync();
     SVH_PHASE(3)
-    // ---- dS_p a, row r
+    // ---- dS_p a, row r, with the g(r) stored by the Sigma phase: entry
+    // (max(r, c), min(r, c)) of the packed triangle (c = r reads a finite
+    // neighbour entry and multiplies a zero difference)
+    const int rd = opaque(r);
+    const int gb_r = g_base(rd);
     if (act && MIX) {
       // float32 terms and accumulation
       float Rf[DM];
@@ -1123,14 +1255,11 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll 2
       for (int c = 0; c < 32; ++c) {
         const float ac = (float)ab[c];
-        float dl[DM], s2 = 0.0f;
+        const int gi = (c < rd) ? g_base(c) + rd : (c > rd ? gb_r + c : 0);
+        const float g = (float)gh[gi];
+        float dl[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) {
-          dl[l] = (float)(x[l] - xs[c * DM + l]);
-          s2 = fmaf(dl[l], dl[l], s2);
-        }
-        float M, g;
-        matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
+        for (int l = 0; l < DM; ++l) dl[l] = (float)(x[l] - xs[c * DM + l]);
         const float w = g * ac;
 #pragma unroll
         for (int l = 0; l < DM; ++l) Rf[l] = fmaf(w, dl[l] * dl[l], Rf[l]);
@@ -1142,14 +1271,11 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll 2
       for (int c = 0; c < 32; ++c) {
         const double ac = ab[c];
-        double dl[DM], s2 = 0.0;
+        const int gi = (c < rd) ? g_base(c) + rd : (c > rd ? gb_r + c : 0);
+        const double g = gh[gi];
+        double dl[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) {
-          dl[l] = x[l] - xs[c * DM + l];
-          s2 = fma(dl[l], dl[l], s2);
-        }
-        double M, g;
-        matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+        for (int l = 0; l < DM; ++l) dl[l] = x[l] - xs[c * DM + l];
         const double w = g * ac;
 #pragma unroll
         for (int l = 0; l < DM; ++l) R[1 + l] = fma(w, dl[l] * dl[l], R[1 + l]);
@@ -1173,7 +1299,8 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
       for (int c = 0; c < kRC; ++c) R[c] = (double)(Rf[c] * df);
     } else {
-      w_sweep<0>(R, S, r);
+      asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
+      w_sweep_dpp(R, S, opaque(r));
 #pragma unroll
       for (int c = 0; c < kRC; ++c) R[c] *= dinv;
     }
