@@ -886,20 +886,6 @@ __device__ __forceinline__ double half_sum(double v) {
   return a0 + a1;
 }
 
-// float v of lane SRC of this lane's half (SRC in 0..31), for both halves at
-// once; all lanes active.  row_newbcast gives every 16-lane row its lane
-// SRC % 16, the row pair swap then hands row SRC / 16 of each half to both of
-// its rows.
-template <int SRC>
-__device__ __forceinline__ float half_bcast_f(float v) {
-  static_assert(SRC >= 0 && SRC < 32, "source lane of a half");
-  const int t = __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + (SRC & 15), 0xF, 0xF,
-                                            false);
-  const auto sw = __builtin_amdgcn_permlane16_swap((unsigned)t, (unsigned)t, false, false);
-  constexpr int o = SRC < 16 ? 0 : 1;
-  return __int_as_float((int)sw[o]);
-}
-
 // LDS doubles per wave of the rows kernel: per half [ab 32 | xs 32*dm], then
 // the packed lower triangles of g(r) of both halves (2 x 496; the product
 // operands Mt, 24 x 33, reuse that space once g is consumed), then misc (16).
@@ -952,6 +938,25 @@ __device__ __forceinline__ void bf4_src(double& a0, double& a1, double& a2, doub
                : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
                : "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(m), "i"(N));
 }
+#define SVH_BFF(acc, src, m, n) \
+  "v_fmac_f32_dpp %" #acc ", %" #src ", -%" #m " row_newbcast:%" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1"
+// float32 forms (the mixed mode's W sweep)
+template <int N>
+__device__ __forceinline__ void bf4f_self(float& a0, float& a1, float& a2, float& a3, float m) {
+  asm volatile("s_nop 1\n\t" SVH_BFF(0, 0, 4, 5) "\n\t" SVH_BFF(1, 1, 4, 5) "\n\t" SVH_BFF(
+                   2, 2, 4, 5) "\n\t" SVH_BFF(3, 3, 4, 5)
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+               : "v"(m), "i"(N));
+}
+template <int N>
+__device__ __forceinline__ void bf4f_src(float& a0, float& a1, float& a2, float& a3, float s0,
+                                         float s1, float s2, float s3, float m) {
+  asm volatile("s_nop 1\n\t" SVH_BFF(0, 4, 8, 9) "\n\t" SVH_BFF(1, 5, 8, 9) "\n\t" SVH_BFF(
+                   2, 6, 8, 9) "\n\t" SVH_BFF(3, 7, 8, 9)
+               : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+               : "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(m), "i"(N));
+}
+#undef SVH_BFF
 #undef SVH_BF
 
 // A copy of a lane index that the compiler cannot see through: each phase
@@ -1086,23 +1091,55 @@ __device__ __forceinline__ void w_sweep_dpp(double (&R)[12], double (&S)[32], in
   wsw_hi<16>(R, S, r);
 }
 
-// w_sweep in float32 (mixed mode): the unit factor's entries rounded to float,
-// float right-hand sides and float accumulation.
+// The same sweep in float32 (mixed mode): the unit factor's entries rounded to
+// float, float right-hand sides and accumulation.
 template <int K>
-__device__ __forceinline__ void w_sweep_f(float (&R)[kRC], const double (&S)[32], int r) {
-  const float l = (r > K) ? (float)S[K] : 0.0f;
-#pragma unroll
-  for (int c0 = 0; c0 < kRC; c0 += 4) {
-    float wk[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) wk[u] = half_bcast_f<K>(R[c0 + u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) R[c0 + u] = fmaf(-l, wk[u], R[c0 + u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) asm volatile("" : "+v"(R[c0 + u]));
+__device__ __forceinline__ void wswf_lo(float (&R)[12], double (&S)[32], int r) {
+  if constexpr (K < 15) {
+    asm volatile("" : "+v"(S[K]));
+    const float m = (r > K && r < 16) ? (float)S[K] : 0.0f;
+    bf4f_self<K>(R[0], R[1], R[2], R[3], m);
+    bf4f_self<K>(R[4], R[5], R[6], R[7], m);
+    bf4f_self<K>(R[8], R[9], R[10], R[11], m);
     __builtin_amdgcn_sched_barrier(0);
+    wswf_lo<K + 1>(R, S, r);
   }
-  if constexpr (K + 1 < 31) w_sweep_f<K + 1>(R, S, r);
+}
+template <int J>
+__device__ __forceinline__ void wswf_from_lo(float (&R)[12], const float (&W)[12], double (&S)[32],
+                                             bool hi_row) {
+  if constexpr (J < 16) {
+    asm volatile("" : "+v"(S[J]));
+    const float m = hi_row ? (float)S[J] : 0.0f;
+    bf4f_src<J>(R[0], R[1], R[2], R[3], W[0], W[1], W[2], W[3], m);
+    bf4f_src<J>(R[4], R[5], R[6], R[7], W[4], W[5], W[6], W[7], m);
+    bf4f_src<J>(R[8], R[9], R[10], R[11], W[8], W[9], W[10], W[11], m);
+    wswf_from_lo<J + 1>(R, W, S, hi_row);
+  }
+}
+template <int K>
+__device__ __forceinline__ void wswf_hi(float (&R)[12], double (&S)[32], int r) {
+  if constexpr (K < 31) {
+    asm volatile("" : "+v"(S[K]));
+    const float m = (r > K) ? (float)S[K] : 0.0f;
+    bf4f_self<K - 16>(R[0], R[1], R[2], R[3], m);
+    bf4f_self<K - 16>(R[4], R[5], R[6], R[7], m);
+    bf4f_self<K - 16>(R[8], R[9], R[10], R[11], m);
+    __builtin_amdgcn_sched_barrier(0);
+    wswf_hi<K + 1>(R, S, r);
+  }
+}
+__device__ __forceinline__ void w_sweep_dpp_f(float (&R)[12], double (&S)[32], int r) {
+  wswf_lo<0>(R, S, r);
+  float W[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(R[c]), __float_as_uint(R[c]),
+                                                     false, false);
+    W[c] = __uint_as_float(sw[0]);
+  }
+  wswf_from_lo<0>(R, W, S, r >= 16);
+  wswf_hi<16>(R, S, r);
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
@@ -1294,7 +1331,8 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       float Rf[kRC];
 #pragma unroll
       for (int c = 0; c < kRC; ++c) Rf[c] = (float)R[c];
-      w_sweep_f<0>(Rf, S, r);
+      asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
+      w_sweep_dpp_f(Rf, S, opaque(r));
       const float df = (float)dinv;
 #pragma unroll
       for (int c = 0; c < kRC; ++c) R[c] = (double)(Rf[c] * df);
