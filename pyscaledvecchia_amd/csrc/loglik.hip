@@ -897,6 +897,10 @@ __host__ __device__ inline int rows_lds_doubles(int dm) {
 // Entry (i, j), i > j, of a strictly-lower 32 x 32 triangle packed by
 // columns: column j holds rows j+1..31 contiguously, so g_base(j) + i.
 __host__ __device__ constexpr int g_base(int j) { return 30 * j - (j * (j - 1)) / 2 - 1; }
+// Coordinate l of point p of a block in its 32 x dm slot: pairs of
+// coordinates interleaved by point, so a lane's two coordinates are one
+// 16-byte read and 32 lanes reading 32 points touch distinct banks.
+__host__ __device__ constexpr int xs_at(int l, int p) { return (l >> 1) * 64 + 2 * p + (l & 1); }
 
 // ---- fused lane broadcasts
 // acc -= src[lane N of this lane's 16-lane row] * m as one v_fmac_f64 with a
@@ -1173,8 +1177,10 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   for (int l = 0; l < DM; ++l) {
     raw[l] = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
     x[l] = (l < d) ? raw[l] * a.inv_lam[l] : 0.0;
-    xs[r * DM + l] = x[l];
+    xs[xs_at(l, r)] = x[l];
   }
+  const int Kb0 = (b0 < a.nblk) ? a.kcnt[b0] + 1 : 1;           // K of both blocks
+  const int Kb1 = (b0 + 1 < a.nblk) ? a.kcnt[b0 + 1] + 1 : 1;
   double bv[1 + kRQ];
   bv[0] = act ? a.y[pt] : 0.0;
   // trend columns: gathered, or formed from the inputs (Z = [1, x] or Z = 1)
@@ -1201,47 +1207,87 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   }
   wave_sync();
   SVH_PHASE(0)
-  // ---- Sigma row r, lower part (upper part zero); g(r) of the lower pairs
-  // to the packed triangle (zero for padding rows) for the dS a phase
-  double S[32];
+  // ---- Sigma and g(r), each unordered pair once.  The 496 lower pairs of
+  // block hh are spread over the whole wave: lane (r, e), e = lane >> 5,
+  // evaluates (r, (r + t) & 31) for t = 8e + 1 .. 8e + 8 (t = 16 for r < 16
+  // only).  Sigma entries go to the packed triangle of block hh and are read
+  // back row by row; g(r) waits in registers and then takes their place
+  // (zero for pairs with a padding point).
+  double gk[16];
   {
-  const int r = opaque(lane_o & 31);
+    const int rr = opaque(lane_o & 31), e = lane_o >> 5;
 #pragma unroll
-  for (int c = 0; c < 32; ++c) {
-    double v = 0.0, gs = 0.0;
-    if (c == r) {
-      v = act ? var * (1.0 + nug) + jit : 1.0;
-    } else if (c < r && act) {
-      if (MIX && a.sig32) {
-        // float32 assembly of the off-diagonal entries (coordinate
-        // differences taken in float64, then rounded)
-        float s2 = 0.0f;
+    for (int hh = 0; hh < 2; ++hh) {
+      const int Kb = hh == 0 ? Kb0 : Kb1;
+      const double* xh = lds + hh * rows_half_doubles(DM) + 32;   // block hh's coordinates
+      double xi[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) {
-          const float t = (float)(x[l] - xs[c * DM + l]);
-          s2 = fmaf(t, t, s2);
+      for (int l = 0; l < DM; ++l) xi[l] = xh[xs_at(l, rr)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = 8 * e + 1 + u;
+        const int c = (rr + t) & 31;
+        const int i = rr > c ? rr : c, j = rr > c ? c : rr;
+        double v = 0.0, gs = 0.0;
+        if (i < Kb) {
+          if (MIX && a.sig32) {
+            // float32 assembly of the off-diagonal entries (coordinate
+            // differences taken in float64, then rounded)
+            float s2 = 0.0f;
+#pragma unroll
+            for (int l = 0; l < DM; ++l) {
+              const float dd = (float)(xi[l] - xh[xs_at(l, c)]);
+              s2 = fmaf(dd, dd, s2);
+            }
+            float M, g;
+            matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
+            v = var * (double)M;
+            gs = (double)g;
+          } else {
+            double s2 = 0.0;
+#pragma unroll
+            for (int l = 0; l < DM; ++l) {
+              const double dd = xi[l] - xh[xs_at(l, c)];
+              s2 = fma(dd, dd, s2);
+            }
+            double M, g;
+            matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+            v = var * M;
+            gs = g;
+          }
         }
-        float M, g;
-        matern_f<NU>(__fsqrt_rn(s2), M, g, a.nu_code);
-        v = var * (double)M;
-        gs = (double)g;
-      } else {
-        double s2 = 0.0;
-#pragma unroll
-        for (int l = 0; l < DM; ++l) {
-          const double t = x[l] - xs[c * DM + l];
-          s2 = fma(t, t, s2);
-        }
-        double M, g;
-        matern_t<NU>(sqrt(s2), M, g, a.nu_code);
-        v = var * M;
-        gs = g;
+        if (t < 16 || rr < 16) gt[hh * kGTri + g_base(j) + i] = v;
+        gk[8 * hh + u] = gs;
+        __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
       }
     }
-    if (c < r) gh[g_base(c) + r] = gs;
-    S[c] = v;
-    __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
   }
+  wave_sync();
+  // ---- Sigma row r of this half's block, lower part (upper part zero)
+  double S[32];
+  {
+    const int rr = opaque(r);
+    const double dg = act ? var * (1.0 + nug) + jit : 1.0;
+#pragma unroll
+    for (int c = 0; c < 31; ++c) {
+      const int gi = g_base(c) + rr;
+      const double v = gh[gi > 0 ? gi : 0];
+      S[c] = (c < rr) ? v : (c == rr ? dg : 0.0);
+    }
+    S[31] = (rr == 31) ? dg : 0.0;
+  }
+  wave_sync();   // every row is read before g takes the triangle
+  {
+    const int rr = opaque(lane_o & 31), e = lane_o >> 5;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = 8 * e + 1 + u;
+        const int c = (rr + t) & 31;
+        const int i = rr > c ? rr : c, j = rr > c ? c : rr;
+        if (t < 16 || rr < 16) gt[hh * kGTri + g_base(j) + i] = gk[8 * hh + u];
+      }
   }
   SVH_PHASE(1)
   // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly,
@@ -1296,7 +1342,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const float g = (float)gh[gi];
         float dl[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) dl[l] = (float)(x[l] - xs[c * DM + l]);
+        for (int l = 0; l < DM; ++l) dl[l] = (float)(x[l] - xs[xs_at(l, c)]);
         const float w = g * ac;
 #pragma unroll
         for (int l = 0; l < DM; ++l) Rf[l] = fmaf(w, dl[l] * dl[l], Rf[l]);
@@ -1312,7 +1358,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const double g = gh[gi];
         double dl[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) dl[l] = x[l] - xs[c * DM + l];
+        for (int l = 0; l < DM; ++l) dl[l] = x[l] - xs[xs_at(l, c)];
         const double w = g * ac;
 #pragma unroll
         for (int l = 0; l < DM; ++l) R[1 + l] = fma(w, dl[l] * dl[l], R[1 + l]);
