@@ -405,6 +405,88 @@ __device__ __forceinline__ void matern_t(double r, double& M, double& g, int nu_
   }
 }
 
+// A value forced into scalar registers at this point (an empty asm with an
+// SGPR operand): constants rebuilt at each use by scalar moves instead of
+// being held in vector registers across the whole kernel.
+__device__ __forceinline__ double sreg(double v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+// e^-c for c >= 0: n = rint(-c log2 e), f = -c - n ln 2 with ln 2 in two
+// parts, e^f by its Taylor polynomial to degree 13 (|f| <= ln 2 / 2, so the
+// truncation is below 5e-18 relative), scaled by 2^n.  Past c = 746 the result
+// is below the smallest subnormal: the clamp lets ldexp return 0.
+__device__ __forceinline__ double exp_neg(double c) {
+  const double x = -fmin(c, 746.0);
+  const double n = rint(x * sreg(0x1.71547652b82fep+0));
+  double f = fma(-n, sreg(0x1.62e42fefa39efp-1), x);
+  f = fma(-n, sreg(0x1.abc9e3b39803fp-56), f);
+  double p = sreg(0x1.6124613a86d09p-33);       // 1/13!
+  p = fma(p, f, sreg(0x1.1eed8eff8d898p-29));   // 1/12!
+  p = fma(p, f, sreg(0x1.ae64567f544e4p-26));   // 1/11!
+  p = fma(p, f, sreg(0x1.27e4fb7789f5cp-22));   // 1/10!
+  p = fma(p, f, sreg(0x1.71de3a556c734p-19));   // 1/9!
+  p = fma(p, f, sreg(0x1.a01a01a01a01ap-16));   // 1/8!
+  p = fma(p, f, sreg(0x1.a01a01a01a01ap-13));   // 1/7!
+  p = fma(p, f, sreg(0x1.6c16c16c16c17p-10));   // 1/6!
+  p = fma(p, f, sreg(0x1.1111111111111p-7));    // 1/5!
+  p = fma(p, f, sreg(0x1.5555555555555p-5));    // 1/4!
+  p = fma(p, f, sreg(0x1.5555555555555p-3));    // 1/3!
+  p = fma(p, f, 0.5);
+  p = fma(p, f, 1.0);
+  p = fma(p, f, 1.0);
+  return ldexp(p, (int)n);
+}
+
+// sqrt(a) for a >= 0 from the hardware reciprocal square root and two
+// correction steps (Goldschmidt iteration: g -> sqrt(a), h -> 1/(2 sqrt(a)));
+// inputs below 2^-900 (squared distances of coincident points) give 0.
+__device__ __forceinline__ double sqrt_pos(double a) {
+  const double y = __builtin_amdgcn_rsq(a);
+  double g = a * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double e = fma(-g, g, a);
+  g = fma(e, h, g);
+  e = fma(-g, g, a);
+  g = fma(e, h, g);
+  return (a > 0x1p-900) ? g : 0.0;
+}
+
+// Matern M(r) and g = M'(r)/r at r = sqrt(s2) for the closed-form
+// smoothness values, with exp_neg and sqrt_pos (the rows kernel's covariance
+// pairs).
+template <int NU>
+__device__ __forceinline__ void matern_s2(double s2, double& M, double& g, int nu_rt = 2) {
+  if constexpr (NU < 0) {
+    if (nu_rt == 0)
+      matern_s2<0>(s2, M, g);
+    else if (nu_rt == 1)
+      matern_s2<1>(s2, M, g);
+    else
+      matern_s2<2>(s2, M, g);
+  } else {
+    const double r = sqrt_pos(s2);
+    if constexpr (NU == 0) {
+      const double e = exp_neg(r);
+      M = e;
+      g = (r > 0.0) ? -e / r : 0.0;
+    } else if constexpr (NU == 1) {
+      const double c = 1.7320508075688772 * r;
+      const double e = exp_neg(c);
+      M = (1.0 + c) * e;
+      g = -3.0 * e;
+    } else {
+      const double c = 2.23606797749979 * r;
+      const double e = exp_neg(c);
+      M = (1.0 + c + c * c * (1.0 / 3.0)) * e;
+      g = -(5.0 / 3.0) * (1.0 + c) * e;
+    }
+  }
+}
+
 // Matern M and g = M'/r in float32 (the mixed mode's covariance assembly and
 // derivative terms): the same closed forms, hardware exp2 and sqrt.
 template <int NU>
@@ -1010,23 +1092,30 @@ __device__ __forceinline__ void rhs_self(double (&V)[12], double m) {
 
 // Rows 16..31 of each half take their updates from the finished rows 0..15:
 // V[c] -= sum_{j<16} l_rj V_j[c] in order j = 0..15 (l_rj = S[j] there).
-template <int J>
-__device__ __forceinline__ void rhs_from_lo(double (&V)[12], const double (&W)[12],
+// Four columns C..C+3 at a time (the copies of the first rows' values, W,
+// for four columns only).
+template <int C, int J>
+__device__ __forceinline__ void rhs_from_lo(double (&V)[12], const double (&W)[4],
                                             double (&S)[32], bool hi_row) {
   if constexpr (J < 16) {
     asm volatile("" : "+v"(S[J]));
     const double m = hi_row ? S[J] : 0.0;
-    bf4_src<J>(V[0], V[1], V[2], V[3], W[0], W[1], W[2], W[3], m);
-    bf4_src<J>(V[4], V[5], V[6], V[7], W[4], W[5], W[6], W[7], m);
-    bf4_src<J>(V[8], V[9], V[10], V[11], W[8], W[9], W[10], W[11], m);
-    rhs_from_lo<J + 1>(V, W, S, hi_row);
+    bf4_src<J>(V[C], V[C + 1], V[C + 2], V[C + 3], W[0], W[1], W[2], W[3], m);
+    rhs_from_lo<C, J + 1>(V, W, S, hi_row);
   }
 }
-__device__ __forceinline__ void rhs_lo_to_hi(double (&V)[12], double (&S)[32], int r) {
-  double W[12];
+template <int C>
+__device__ __forceinline__ void rhs_lo_to_hi4(double (&V)[12], double (&S)[32], int r) {
+  double W[4];
 #pragma unroll
-  for (int c = 0; c < 12; ++c) W[c] = from_lo_row(V[c]);
-  rhs_from_lo<0>(V, W, S, r >= 16);
+  for (int u = 0; u < 4; ++u) W[u] = from_lo_row(V[C + u]);
+  rhs_from_lo<C, 0>(V, W, S, r >= 16);
+  __builtin_amdgcn_sched_barrier(0);
+}
+__device__ __forceinline__ void rhs_lo_to_hi(double (&V)[12], double (&S)[32], int r) {
+  rhs_lo_to_hi4<0>(V, S, r);
+  rhs_lo_to_hi4<4>(V, S, r);
+  rhs_lo_to_hi4<8>(V, S, r);
 }
 
 // Step k of the unit LDL^T factorisation of the rows kernel, with the rows of
@@ -1172,32 +1261,14 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   const bool act = bvalid && r < K;
   int pt = 0;
   if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
-  double x[DM], raw[DM];
 #pragma unroll
   for (int l = 0; l < DM; ++l) {
-    raw[l] = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
-    x[l] = (l < d) ? raw[l] * a.inv_lam[l] : 0.0;
-    xs[xs_at(l, r)] = x[l];
+    const double raw = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
+    xs[xs_at(l, r)] = (l < d) ? raw * a.inv_lam[l] : 0.0;
   }
   const int Kb0 = (b0 < a.nblk) ? a.kcnt[b0] + 1 : 1;           // K of both blocks
   const int Kb1 = (b0 + 1 < a.nblk) ? a.kcnt[b0 + 1] + 1 : 1;
-  double bv[1 + kRQ];
-  bv[0] = act ? a.y[pt] : 0.0;
-  // trend columns: gathered, or formed from the inputs (Z = [1, x] or Z = 1)
-  const int zmode = a.zmode;
-#pragma unroll
-  for (int c = 0; c < kRQ; ++c) {
-    double z = 0.0;
-    if (act && c < q) {
-      if (zmode == 0)
-        z = a.Z[(int64_t)pt * q + c];
-      else if (c == 0)
-        z = 1.0;
-      else
-        z = raw[c - 1 < DM ? c - 1 : 0];
-    }
-    bv[1 + c] = z;
-  }
+  const double yv = act ? a.y[pt] : 0.0;
   const double var = a.var, nug = a.nug;
   double jit = 0.0;
   if (lev > 0) {
@@ -1251,7 +1322,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
               s2 = fma(dd, dd, s2);
             }
             double M, g;
-            matern_t<NU>(sqrt(s2), M, g, a.nu_code);
+            matern_s2<NU>(s2, M, g, a.nu_code);
             v = var * M;
             gs = g;
           }
@@ -1260,6 +1331,29 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         gk[8 * hh + u] = gs;
         __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
       }
+    }
+  }
+  // right-hand sides [y Z] of row r (formed here, not held through the pair
+  // phase): trend columns gathered, or formed from the inputs (Z = [1, x] or
+  // Z = 1), the inputs read again (an opaque index: not merged with the
+  // gather's loads, whose values would then stay live)
+  double bv[1 + kRQ];
+  {
+    bv[0] = yv;
+    const int zmode = a.zmode;
+    const int p2 = opaque(pt);
+#pragma unroll
+    for (int c = 0; c < kRQ; ++c) {
+      double z = 0.0;
+      if (act && c < q) {
+        if (zmode == 0)
+          z = a.Z[(int64_t)p2 * q + c];
+        else if (c == 0)
+          z = 1.0;
+        else if (c - 1 < DM)
+          z = a.X[(int64_t)p2 * d + (c - 1)];
+      }
+      bv[1 + c] = z;
     }
   }
   wave_sync();
@@ -1330,6 +1424,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     // neighbour entry and multiplies a zero difference)
     const int rd = opaque(r);
     const int gb_r = g_base(rd);
+    double x[DM];
+#pragma unroll
+    for (int l = 0; l < DM; ++l) x[l] = xs[xs_at(l, rd)];
     if (act && MIX) {
       // float32 terms and accumulation
       float Rf[DM];
