@@ -979,6 +979,8 @@ __host__ __device__ inline int rows_lds_doubles(int dm) {
 // Entry (i, j), i > j, of a strictly-lower 32 x 32 triangle packed by
 // columns: column j holds rows j+1..31 contiguously, so g_base(j) + i.
 __host__ __device__ constexpr int g_base(int j) { return 30 * j - (j * (j - 1)) / 2 - 1; }
+// The same for run-time indices 0 <= j <= 31 (24-bit multiply, full rate).
+__device__ __forceinline__ int g_base_rt(int j) { return (int)(__umul24((unsigned)j, (unsigned)(61 - j)) >> 1) - 1; }
 // Coordinate l of point p of a block in its 32 x dm slot: pairs of
 // coordinates interleaved by point, so a lane's two coordinates are one
 // 16-byte read and 32 lanes reading 32 points touch distinct banks.
@@ -1299,8 +1301,8 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const int t = 8 * e + 1 + u;
         const int c = (rr + t) & 31;
         const int i = rr > c ? rr : c, j = rr > c ? c : rr;
-        double v = 0.0, gs = 0.0;
-        if (i < Kb) {
+        double v, gs;
+        {
           if (MIX && a.sig32) {
             // float32 assembly of the off-diagonal entries (coordinate
             // differences taken in float64, then rounded)
@@ -1327,7 +1329,11 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
             gs = g;
           }
         }
-        if (t < 16 || rr < 16) gt[hh * kGTri + g_base(j) + i] = v;
+        // pairs with a padding point (zero coordinates, finite values) are zero
+        const bool real = i < Kb;
+        v = real ? v : 0.0;
+        gs = real ? gs : 0.0;
+        if (t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = v;
         gk[8 * hh + u] = gs;
         __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
       }
@@ -1380,7 +1386,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const int t = 8 * e + 1 + u;
         const int c = (rr + t) & 31;
         const int i = rr > c ? rr : c, j = rr > c ? c : rr;
-        if (t < 16 || rr < 16) gt[hh * kGTri + g_base(j) + i] = gk[8 * hh + u];
+        if (t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = gk[8 * hh + u];
       }
   }
   SVH_PHASE(1)
@@ -1427,7 +1433,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     double x[DM];
 #pragma unroll
     for (int l = 0; l < DM; ++l) x[l] = xs[xs_at(l, rd)];
-    if (act && MIX) {
+    // every lane runs the sums (padding rows have zero g and a), results kept
+    // for the block's rows only
+    if constexpr (MIX) {
       // float32 terms and accumulation
       float Rf[DM];
 #pragma unroll
@@ -1446,9 +1454,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
-      for (int l = 0; l < DM; ++l) R[1 + l] = -var * (double)Rf[l];
-    } else if (act) {
-#pragma unroll 2
+      for (int l = 0; l < DM; ++l) R[1 + l] = act ? -var * (double)Rf[l] : 0.0;
+    } else {
+#pragma unroll
       for (int c = 0; c < 32; ++c) {
         const double ac = ab[c];
         const int gi = (c < rd) ? g_base(c) + rd : (c > rd ? gb_r + c : 0);
@@ -1462,12 +1470,10 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
-      for (int l = 0; l < DM; ++l) R[1 + l] *= -var;
+      for (int l = 0; l < DM; ++l) R[1 + l] = act ? R[1 + l] * -var : 0.0;
     }
-    if (act) {
-      R[0] = ((r == K - 1) ? pK * dK : 0.0) - jit * av;
-      R[kRC - 1] = var * nug * av;
-    }
+    R[0] = act ? ((r == K - 1) ? pK * dK : 0.0) - jit * av : 0.0;
+    R[kRC - 1] = act ? var * nug * av : 0.0;
     SVH_PHASE(4)
     // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
     if constexpr (MIX) {
