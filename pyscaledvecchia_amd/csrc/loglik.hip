@@ -928,7 +928,6 @@ void launch_split_nu(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mi
 //     -var sum_c g_rc (x_rl - x_cl)^2 a_c; nugget: var tau a;
 //   * W = L^-1 (dS a) by a column sweep, then the products W^T [W | L^-1 y |
 //     L^-1 Z] on the matrix cores, block by block (as in the split kernel).
-constexpr int kRQ = 11, kRC = 12;
 
 // The rows kernels run one wave per workgroup, and a wave's LDS operations
 // complete in issue order: exchanging values between its lanes through LDS
@@ -972,6 +971,13 @@ __device__ __forceinline__ double half_sum(double v) {
 // the packed lower triangles of g(r) of both halves (2 x 496; the product
 // operands Mt, 24 x 33, reuse that space once g is consumed), then misc (16).
 constexpr int kGTri = 496;   // 32 * 31 / 2 strictly-lower entries
+// Coordinate registers of the rows kernel for d inputs and q trend columns:
+// the kernel carries DM + 2 right-hand-side columns, which must hold both
+// [y Z] (1 + q) and the d + 2 derivative columns.
+__host__ __device__ inline int rows_dm(int d, int q) {
+  const int k = d > q - 1 ? d : q - 1;
+  return k <= 4 ? 4 : (k <= 8 ? 8 : 10);
+}
 __host__ __device__ inline int rows_half_doubles(int dm) { return 32 + 32 * dm; }
 __host__ __device__ inline int rows_lds_doubles(int dm) {
   return 2 * rows_half_doubles(dm) + 2 * kGTri + 16;
@@ -1028,6 +1034,19 @@ __device__ __forceinline__ void bf4_src(double& a0, double& a1, double& a2, doub
 }
 #define SVH_BFF(acc, src, m, n) \
   "v_fmac_f32_dpp %" #acc ", %" #src ", -%" #m " row_newbcast:%" #n " row_mask:0xf bank_mask:0xf bound_ctrl:1"
+// two accumulators, each its own source / two separate sources, one lane
+template <int N>
+__device__ __forceinline__ void bf2_self(double& a0, double& a1, double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 0, 2, 3) "\n\t" SVH_BF(1, 1, 2, 3)
+               : "+v"(a0), "+v"(a1)
+               : "v"(m), "i"(N));
+}
+template <int N>
+__device__ __forceinline__ void bf2_src(double& a0, double& a1, double s0, double s1, double m) {
+  asm volatile("s_nop 1\n\t" SVH_BF(0, 2, 4, 5) "\n\t" SVH_BF(1, 3, 4, 5)
+               : "+v"(a0), "+v"(a1)
+               : "v"(s0), "v"(s1), "v"(m), "i"(N));
+}
 // float32 forms (the mixed mode's W sweep)
 template <int N>
 __device__ __forceinline__ void bf4f_self(float& a0, float& a1, float& a2, float& a3, float m) {
@@ -1043,6 +1062,18 @@ __device__ __forceinline__ void bf4f_src(float& a0, float& a1, float& a2, float&
                    2, 6, 8, 9) "\n\t" SVH_BFF(3, 7, 8, 9)
                : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
                : "v"(s0), "v"(s1), "v"(s2), "v"(s3), "v"(m), "i"(N));
+}
+template <int N>
+__device__ __forceinline__ void bf2f_self(float& a0, float& a1, float m) {
+  asm volatile("s_nop 1\n\t" SVH_BFF(0, 0, 2, 3) "\n\t" SVH_BFF(1, 1, 2, 3)
+               : "+v"(a0), "+v"(a1)
+               : "v"(m), "i"(N));
+}
+template <int N>
+__device__ __forceinline__ void bf2f_src(float& a0, float& a1, float s0, float s1, float m) {
+  asm volatile("s_nop 1\n\t" SVH_BFF(0, 2, 4, 5) "\n\t" SVH_BFF(1, 3, 4, 5)
+               : "+v"(a0), "+v"(a1)
+               : "v"(s0), "v"(s1), "v"(m), "i"(N));
 }
 #undef SVH_BFF
 #undef SVH_BF
@@ -1084,40 +1115,75 @@ __device__ __forceinline__ void fac_trail(double (&S)[32], double src, double m)
   }
 }
 
-// Twelve right-hand-side columns: V[c] -= V[c](lane N) * m
-template <int N>
-__device__ __forceinline__ void rhs_self(double (&V)[12], double m) {
-  bf4_self<N>(V[0], V[1], V[2], V[3], m);
-  bf4_self<N>(V[4], V[5], V[6], V[7], m);
-  bf4_self<N>(V[8], V[9], V[10], V[11], m);
+// NC right-hand-side columns (NC even): V[c] -= V[c](lane N) * m, in groups
+// of four and a last pair.
+template <int N, int C, int NC, class T>
+__device__ __forceinline__ void rhs_self_from(T (&V)[NC], T m) {
+  if constexpr (NC - C >= 4) {
+    if constexpr (sizeof(T) == 8)
+      bf4_self<N>(V[C], V[C + 1], V[C + 2], V[C + 3], m);
+    else
+      bf4f_self<N>(V[C], V[C + 1], V[C + 2], V[C + 3], m);
+    rhs_self_from<N, C + 4, NC>(V, m);
+  } else if constexpr (NC - C >= 2) {
+    if constexpr (sizeof(T) == 8)
+      bf2_self<N>(V[C], V[C + 1], m);
+    else
+      bf2f_self<N>(V[C], V[C + 1], m);
+    rhs_self_from<N, C + 2, NC>(V, m);
+  }
+}
+template <int N, int NC, class T>
+__device__ __forceinline__ void rhs_self(T (&V)[NC], T m) {
+  static_assert(NC % 2 == 0, "even column count");
+  rhs_self_from<N, 0, NC>(V, m);
 }
 
 // Rows 16..31 of each half take their updates from the finished rows 0..15:
 // V[c] -= sum_{j<16} l_rj V_j[c] in order j = 0..15 (l_rj = S[j] there).
-// Four columns C..C+3 at a time (the copies of the first rows' values, W,
-// for four columns only).
-template <int C, int J>
-__device__ __forceinline__ void rhs_from_lo(double (&V)[12], const double (&W)[4],
-                                            double (&S)[32], bool hi_row) {
+// Four (or two) columns C.. at a time: W holds the first rows' values of
+// those columns copied to both 16-lane rows.
+template <int C, int G, int J, int NC, class T>
+__device__ __forceinline__ void rhs_from_lo(T (&V)[NC], const T (&W)[4], double (&S)[32],
+                                            bool hi_row) {
   if constexpr (J < 16) {
     asm volatile("" : "+v"(S[J]));
-    const double m = hi_row ? S[J] : 0.0;
-    bf4_src<J>(V[C], V[C + 1], V[C + 2], V[C + 3], W[0], W[1], W[2], W[3], m);
-    rhs_from_lo<C, J + 1>(V, W, S, hi_row);
+    const T m = hi_row ? (T)S[J] : (T)0;
+    if constexpr (G == 4) {
+      if constexpr (sizeof(T) == 8)
+        bf4_src<J>(V[C], V[C + 1], V[C + 2], V[C + 3], W[0], W[1], W[2], W[3], m);
+      else
+        bf4f_src<J>(V[C], V[C + 1], V[C + 2], V[C + 3], W[0], W[1], W[2], W[3], m);
+    } else {
+      if constexpr (sizeof(T) == 8)
+        bf2_src<J>(V[C], V[C + 1], W[0], W[1], m);
+      else
+        bf2f_src<J>(V[C], V[C + 1], W[0], W[1], m);
+    }
+    rhs_from_lo<C, G, J + 1, NC>(V, W, S, hi_row);
   }
 }
-template <int C>
-__device__ __forceinline__ void rhs_lo_to_hi4(double (&V)[12], double (&S)[32], int r) {
-  double W[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) W[u] = from_lo_row(V[C + u]);
-  rhs_from_lo<C, 0>(V, W, S, r >= 16);
-  __builtin_amdgcn_sched_barrier(0);
+__device__ __forceinline__ double lo_row_both(double v) { return from_lo_row(v); }
+__device__ __forceinline__ float lo_row_both(float v) {
+  const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                   false);
+  return __uint_as_float(sw[0]);
 }
-__device__ __forceinline__ void rhs_lo_to_hi(double (&V)[12], double (&S)[32], int r) {
-  rhs_lo_to_hi4<0>(V, S, r);
-  rhs_lo_to_hi4<4>(V, S, r);
-  rhs_lo_to_hi4<8>(V, S, r);
+template <int C, int NC, class T>
+__device__ __forceinline__ void rhs_lo_to_hi_from(T (&V)[NC], double (&S)[32], int r) {
+  if constexpr (C < NC) {
+    constexpr int G = (NC - C >= 4) ? 4 : 2;
+    T W[4];
+#pragma unroll
+    for (int u = 0; u < G; ++u) W[u] = lo_row_both(V[C + u]);
+    rhs_from_lo<C, G, 0, NC>(V, W, S, r >= 16);
+    __builtin_amdgcn_sched_barrier(0);
+    rhs_lo_to_hi_from<C + G, NC>(V, S, r);
+  }
+}
+template <int NC, class T>
+__device__ __forceinline__ void rhs_lo_to_hi(T (&V)[NC], double (&S)[32], int r) {
+  rhs_lo_to_hi_from<0, NC>(V, S, r);
 }
 
 // Step k of the unit LDL^T factorisation of the rows kernel, with the rows of
@@ -1130,8 +1196,8 @@ __device__ __forceinline__ void rhs_lo_to_hi(double (&V)[12], double (&S)[32], i
 // rows 16..31 take steps 0..15 together after step 15 (rhs_lo_to_hi), then
 // steps 16..30 -- the same operations in the same order per row as a plain
 // right-looking sweep.
-template <int k>
-__device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[12], double& pr,
+template <int k, int NC>
+__device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[NC], double& pr,
                                          bool& fail, int r, int K) {
   double src;
   if constexpr (k < 16)
@@ -1157,90 +1223,44 @@ __device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[12], doub
   if constexpr (k + 1 < 32) fac_step<k + 1>(S, bv, pr, fail, r, K);
 }
 
-// W = L'^-1 R for the rows kernel's twelve columns (unit lower L' in S):
+// W = L'^-1 R for the rows kernel's NC columns (unit lower L' in S), in
+// float64 or float32 (mixed mode: the factor's entries rounded to float):
 // rows 0..15 with steps 0..14, rows 16..31 with steps 0..15 from the finished
-// first rows, then steps 16..30.
-// The multipliers are formed at their step (S[K] passed through an empty asm
-// there): computed ahead, all 31 would be live at once.
-template <int K>
-__device__ __forceinline__ void wsw_lo(double (&R)[12], double (&S)[32], int r) {
+// first rows, then steps 16..30.  The multipliers are formed at their step
+// (S[K] passed through an empty asm there): computed ahead, all 31 would be
+// live at once.
+template <int K, int NC, class T>
+__device__ __forceinline__ void wsw_lo(T (&R)[NC], double (&S)[32], int r) {
   if constexpr (K < 15) {
     asm volatile("" : "+v"(S[K]));
-    rhs_self<K>(R, (r > K && r < 16) ? S[K] : 0.0);
+    rhs_self<K>(R, (r > K && r < 16) ? (T)S[K] : (T)0);
     __builtin_amdgcn_sched_barrier(0);
     wsw_lo<K + 1>(R, S, r);
   }
 }
-template <int K>
-__device__ __forceinline__ void wsw_hi(double (&R)[12], double (&S)[32], int r) {
+template <int K, int NC, class T>
+__device__ __forceinline__ void wsw_hi(T (&R)[NC], double (&S)[32], int r) {
   if constexpr (K < 31) {
     asm volatile("" : "+v"(S[K]));
-    rhs_self<K - 16>(R, (r > K) ? S[K] : 0.0);
+    rhs_self<K - 16>(R, (r > K) ? (T)S[K] : (T)0);
     __builtin_amdgcn_sched_barrier(0);
     wsw_hi<K + 1>(R, S, r);
   }
 }
-__device__ __forceinline__ void w_sweep_dpp(double (&R)[12], double (&S)[32], int r) {
+template <int NC, class T>
+__device__ __forceinline__ void w_sweep_dpp(T (&R)[NC], double (&S)[32], int r) {
   wsw_lo<0>(R, S, r);
   rhs_lo_to_hi(R, S, r);
   wsw_hi<16>(R, S, r);
-}
-
-// The same sweep in float32 (mixed mode): the unit factor's entries rounded to
-// float, float right-hand sides and accumulation.
-template <int K>
-__device__ __forceinline__ void wswf_lo(float (&R)[12], double (&S)[32], int r) {
-  if constexpr (K < 15) {
-    asm volatile("" : "+v"(S[K]));
-    const float m = (r > K && r < 16) ? (float)S[K] : 0.0f;
-    bf4f_self<K>(R[0], R[1], R[2], R[3], m);
-    bf4f_self<K>(R[4], R[5], R[6], R[7], m);
-    bf4f_self<K>(R[8], R[9], R[10], R[11], m);
-    __builtin_amdgcn_sched_barrier(0);
-    wswf_lo<K + 1>(R, S, r);
-  }
-}
-template <int J>
-__device__ __forceinline__ void wswf_from_lo(float (&R)[12], const float (&W)[12], double (&S)[32],
-                                             bool hi_row) {
-  if constexpr (J < 16) {
-    asm volatile("" : "+v"(S[J]));
-    const float m = hi_row ? (float)S[J] : 0.0f;
-    bf4f_src<J>(R[0], R[1], R[2], R[3], W[0], W[1], W[2], W[3], m);
-    bf4f_src<J>(R[4], R[5], R[6], R[7], W[4], W[5], W[6], W[7], m);
-    bf4f_src<J>(R[8], R[9], R[10], R[11], W[8], W[9], W[10], W[11], m);
-    wswf_from_lo<J + 1>(R, W, S, hi_row);
-  }
-}
-template <int K>
-__device__ __forceinline__ void wswf_hi(float (&R)[12], double (&S)[32], int r) {
-  if constexpr (K < 31) {
-    asm volatile("" : "+v"(S[K]));
-    const float m = (r > K) ? (float)S[K] : 0.0f;
-    bf4f_self<K - 16>(R[0], R[1], R[2], R[3], m);
-    bf4f_self<K - 16>(R[4], R[5], R[6], R[7], m);
-    bf4f_self<K - 16>(R[8], R[9], R[10], R[11], m);
-    __builtin_amdgcn_sched_barrier(0);
-    wswf_hi<K + 1>(R, S, r);
-  }
-}
-__device__ __forceinline__ void w_sweep_dpp_f(float (&R)[12], double (&S)[32], int r) {
-  wswf_lo<0>(R, S, r);
-  float W[12];
-#pragma unroll
-  for (int c = 0; c < 12; ++c) {
-    const auto sw = __builtin_amdgcn_permlane16_swap(__float_as_uint(R[c]), __float_as_uint(R[c]),
-                                                     false, false);
-    W[c] = __uint_as_float(sw[0]);
-  }
-  wswf_from_lo<0>(R, W, S, r >= 16);
-  wswf_hi<16>(R, S, r);
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
 __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* lds,
                                           double (&accF)[4], double (&accX)[3],
                                           unsigned long long (&ph)[8]) {
+  // right-hand-side columns: [y Z] with q <= DM + 1 trend columns, and the
+  // P = d + 2 <= DM + 2 derivative columns [sigma^2, ranges, nugget]
+  constexpr int NC = DM + 2;
   unsigned long long ph_t = 0;
   if constexpr (PROF) ph_t = __builtin_amdgcn_s_memtime();
   int lane_o = threadIdx.x;
@@ -1343,13 +1363,13 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   // phase): trend columns gathered, or formed from the inputs (Z = [1, x] or
   // Z = 1), the inputs read again (an opaque index: not merged with the
   // gather's loads, whose values would then stay live)
-  double bv[1 + kRQ];
+  double bv[NC];
   {
     bv[0] = yv;
     const int zmode = a.zmode;
     const int p2 = opaque(pt);
 #pragma unroll
-    for (int c = 0; c < kRQ; ++c) {
+    for (int c = 0; c < NC - 1; ++c) {
       double z = 0.0;
       if (act && c < q) {
         if (zmode == 0)
@@ -1398,7 +1418,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   fac_step<0>(S, bv, pr, fail, opaque(r), K);
   const double dinv = rsqrt_nr(pr);   // 1 / L_rr
 #pragma unroll
-  for (int c = 0; c < 1 + kRQ; ++c) bv[c] *= dinv;   // rows of L^-1 [y Z]
+  for (int c = 0; c < NC; ++c) bv[c] *= dinv;   // rows of L^-1 [y Z]
   // L_KK of this half (from lane K-1)
   if (r == K - 1) {
     misc[8 * h + 1] = pr;
@@ -1408,9 +1428,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   wave_sync();
   const double pK = misc[8 * h + 1], dK = misc[8 * h + 2];
   SVH_PHASE(2)
-  double R[kRC];
+  double R[NC];
 #pragma unroll
-  for (int c = 0; c < kRC; ++c) R[c] = 0.0;
+  for (int c = 0; c < NC; ++c) R[c] = 0.0;
   if (grad) {
     // ---- t = L'^-T e_{K-1}, a = t / L_KK
     const int ra = opaque(r);
@@ -1473,23 +1493,23 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       for (int l = 0; l < DM; ++l) R[1 + l] = act ? R[1 + l] * -var : 0.0;
     }
     R[0] = act ? ((r == K - 1) ? pK * dK : 0.0) - jit * av : 0.0;
-    R[kRC - 1] = act ? var * nug * av : 0.0;
+    R[NC - 1] = act ? var * nug * av : 0.0;
     SVH_PHASE(4)
     // ---- W = L^-1 (dS a): unit solve with L', then scale by 1/L_rr
     if constexpr (MIX) {
-      float Rf[kRC];
+      float Rf[NC];
 #pragma unroll
-      for (int c = 0; c < kRC; ++c) Rf[c] = (float)R[c];
+      for (int c = 0; c < NC; ++c) Rf[c] = (float)R[c];
       asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
-      w_sweep_dpp_f(Rf, S, opaque(r));
+      w_sweep_dpp(Rf, S, opaque(r));
       const float df = (float)dinv;
 #pragma unroll
-      for (int c = 0; c < kRC; ++c) R[c] = (double)(Rf[c] * df);
+      for (int c = 0; c < NC; ++c) R[c] = (double)(Rf[c] * df);
     } else {
       asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
       w_sweep_dpp(R, S, opaque(r));
 #pragma unroll
-      for (int c = 0; c < kRC; ++c) R[c] *= dinv;
+      for (int c = 0; c < NC; ++c) R[c] *= dinv;
     }
     SVH_PHASE(5)
   }
@@ -1507,13 +1527,13 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       if (grad) {
         Mt[0 * 33 + r] = R[0];
 #pragma unroll
-        for (int l = 0; l < kSD; ++l)
+        for (int l = 0; l < DM; ++l)
           if (l < d) Mt[(1 + l) * 33 + r] = R[1 + l];
-        Mt[(d + 1) * 33 + r] = R[kRC - 1];
+        Mt[(d + 1) * 33 + r] = R[NC - 1];
       }
       Mt[cyl * 33 + r] = bv[0];
 #pragma unroll
-      for (int c = 0; c < kRQ; ++c)
+      for (int c = 0; c < NC - 1; ++c)
         if (c < q) Mt[(cyl + 1 + c) * 33 + r] = bv[1 + c];
     }
     wave_sync();
@@ -2647,7 +2667,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
       const Launch& L = *static_cast<Launch*>(u);
       const Cand& c = (*L.cands)[i];
       if (c.kind >= 2) {
-        const int dm = L.a->d <= 4 ? 4 : (L.a->d <= 8 ? 8 : 10);
+        const int dm = rows_dm(L.a->d, L.a->q);
         switch (L.nu_code) {
           case 0: launch_rows_nu<0>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
           case 1: launch_rows_nu<1>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
@@ -2860,7 +2880,8 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
       B.goff[c][g] = (int)((int64_t)htiles.size() - B.toff[c]);
       for (int i = 0; i < ndata; ++i) {
         const DSDesc& D = B.desc[i];
-        const int gi = D.d <= 4 ? 0 : (D.d <= 8 ? 1 : 2);
+        const int gdm = rows_dm(D.d, D.q);
+        const int gi = gdm == 4 ? 0 : (gdm == 8 ? 1 : 2);
         if (gi != g) continue;
         const int np = (int)((D.nblk + 1) / 2);
         hdtile[((size_t)c * ndata + i) * 2] = (int)((int64_t)htiles.size() - B.toff[c]);
