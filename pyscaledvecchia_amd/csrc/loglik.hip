@@ -405,12 +405,24 @@ __device__ __forceinline__ void matern_t(double r, double& M, double& g, int nu_
   }
 }
 
-// A value forced into scalar registers at this point (an empty asm with an
-// SGPR operand): constants rebuilt at each use by scalar moves instead of
-// being held in vector registers across the whole kernel.
-__device__ __forceinline__ double sreg(double v) {
-  asm volatile("" : "+s"(v));
-  return v;
+// A double constant built in scalar registers where it is used (two scalar
+// moves in a volatile asm, BITS = its IEEE bit pattern): neither hoisted into
+// vector registers nor kept live across the kernel.
+template <unsigned long long BITS>
+__device__ __forceinline__ double sconst() {
+  unsigned lo, hi;
+  asm volatile("s_mov_b32 %0, %2\n\ts_mov_b32 %1, %3"
+               : "=s"(lo), "=s"(hi)
+               : "i"((unsigned)(BITS & 0xffffffffull)), "i"((unsigned)(BITS >> 32)));
+  return __hiloint2double((int)hi, (int)lo);
+}
+
+// a * b + c with c in scalar registers as one v_fma_f64 (left to itself the
+// compiler copies c into vector registers for a two-operand v_fmac_f64).
+__device__ __forceinline__ double fma_s(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+  return r;
 }
 
 // e^-c for c >= 0: n = rint(-c log2 e), f = -c - n ln 2 with ln 2 in two
@@ -419,20 +431,19 @@ __device__ __forceinline__ double sreg(double v) {
 // is below the smallest subnormal: the clamp lets ldexp return 0.
 __device__ __forceinline__ double exp_neg(double c) {
   const double x = -fmin(c, 746.0);
-  const double n = rint(x * sreg(0x1.71547652b82fep+0));
-  double f = fma(-n, sreg(0x1.62e42fefa39efp-1), x);
-  f = fma(-n, sreg(0x1.abc9e3b39803fp-56), f);
-  double p = sreg(0x1.6124613a86d09p-33);       // 1/13!
-  p = fma(p, f, sreg(0x1.1eed8eff8d898p-29));   // 1/12!
-  p = fma(p, f, sreg(0x1.ae64567f544e4p-26));   // 1/11!
-  p = fma(p, f, sreg(0x1.27e4fb7789f5cp-22));   // 1/10!
-  p = fma(p, f, sreg(0x1.71de3a556c734p-19));   // 1/9!
-  p = fma(p, f, sreg(0x1.a01a01a01a01ap-16));   // 1/8!
-  p = fma(p, f, sreg(0x1.a01a01a01a01ap-13));   // 1/7!
-  p = fma(p, f, sreg(0x1.6c16c16c16c17p-10));   // 1/6!
-  p = fma(p, f, sreg(0x1.1111111111111p-7));    // 1/5!
-  p = fma(p, f, sreg(0x1.5555555555555p-5));    // 1/4!
-  p = fma(p, f, sreg(0x1.5555555555555p-3));    // 1/3!
+  const double n = rint(x * sconst<0x3FF71547652B82FEull>());        // log2 e
+  double f = fma(-n, sconst<0x3FE62E42FEFA39EFull>(), x);            // ln 2 (high part)
+  f = fma(-n, sconst<0x3C7ABC9E3B39803Full>(), f);                   // ln 2 (low part)
+  double p = fma_s(sconst<0x3DE6124613A86D09ull>(), f, sconst<0x3E21EED8EFF8D898ull>());  // 1/13!, 1/12!
+  p = fma_s(p, f, sconst<0x3E5AE64567F544E4ull>());                  // 1/11!
+  p = fma_s(p, f, sconst<0x3E927E4FB7789F5Cull>());                  // 1/10!
+  p = fma_s(p, f, sconst<0x3EC71DE3A556C734ull>());                  // 1/9!
+  p = fma_s(p, f, sconst<0x3EFA01A01A01A01Aull>());                  // 1/8!
+  p = fma_s(p, f, sconst<0x3F2A01A01A01A01Aull>());                  // 1/7!
+  p = fma_s(p, f, sconst<0x3F56C16C16C16C17ull>());                  // 1/6!
+  p = fma_s(p, f, sconst<0x3F81111111111111ull>());                  // 1/5!
+  p = fma_s(p, f, sconst<0x3FA5555555555555ull>());                  // 1/4!
+  p = fma_s(p, f, sconst<0x3FC5555555555555ull>());                  // 1/3!
   p = fma(p, f, 0.5);
   p = fma(p, f, 1.0);
   p = fma(p, f, 1.0);
@@ -939,25 +950,32 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A double moved between lanes by a DPP control (both dwords), all lanes active.
+// A double moved between lanes by a DPP control (both dwords), all lanes
+// active and every source lane valid (no old value to keep).
 template <int CTRL>
 __device__ __forceinline__ double dpp_double(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum over each half-wave (32 lanes), every lane gets its half's sum; all
-// lanes active.  VALU lane moves only: quad permutes (lane ^ 1, lane ^ 2),
-// rotations by 4 and 8 inside each 16-lane row, then the row pair swap of
-// v_permlane16_swap (rows 0/1 and 2/3), whose two outputs summed give
-// row 0 + row 1 (row 2 + row 3) in every lane of the pair.
-__device__ __forceinline__ double half_sum(double v) {
+// Sum over each 16-lane row, every lane gets its row's sum: quad permutes
+// (lane ^ 1, lane ^ 2) and rotations by 4 and 8 inside the row.
+__device__ __forceinline__ double row_sum(double v) {
   v += dpp_double<0xB1>(v);    // quad_perm [1,0,3,2]
   v += dpp_double<0x4E>(v);    // quad_perm [2,3,0,1]
   v += dpp_double<0x124>(v);   // row_ror:4
   v += dpp_double<0x128>(v);   // row_ror:8
+  return v;
+}
+
+// Sum over each half-wave (32 lanes), every lane gets its half's sum: the row
+// sums, then the row pair swap of v_permlane16_swap (rows 0/1 and 2/3),
+// whose two outputs summed give row 0 + row 1 (row 2 + row 3) in every lane
+// of the pair.
+__device__ __forceinline__ double half_sum(double v) {
+  v = row_sum(v);
   const long long b = __double_as_longlong(v);
   const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
   const auto sl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
@@ -1435,8 +1453,15 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     // ---- t = L'^-T e_{K-1}, a = t / L_KK
     const int ra = opaque(r);
     double t = (ra == K - 1) ? 1.0 : 0.0;
+    // for c >= 16 only rows 16..31 (the second 16-lane row) contribute
 #pragma unroll
-    for (int c = 30; c >= 0; --c) {
+    for (int c = 30; c >= 16; --c) {
+      const double s = row_sum((ra > c) ? S[c] * t : 0.0);
+      if (ra == c) t -= s;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 15; c >= 0; --c) {
       const double s = half_sum((ra > c) ? S[c] * t : 0.0);
       if (ra == c) t -= s;
       __builtin_amdgcn_sched_barrier(0);
