@@ -475,12 +475,16 @@ def main():
     def timed_evals(precision):
         for _ in range(args.warmup):
             st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
-        ctx.timing(True)
-        ctx.timing_reset()
+        # the timed steps run without the library's event timers; the pass-1
+        # kernel duration comes from a second run of the same steps with them
         t0 = time.perf_counter()
         for _ in range(args.steps):
             res = st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
         t = time.perf_counter() - t0     # every call synchronises on return
+        ctx.timing(True)
+        ctx.timing_reset()
+        for _ in range(args.steps):
+            st(theta, 2.5, var_penalty=1.0, log_var_target=lvt, precision=precision)
         ctx.timing(False)
         served = [k for k in ("ll_pass1_rows", "ll_pass1_split", "ll_pass1")
                   if ctx.timing_read("kernel:" + k)[1] > 0]
@@ -554,12 +558,14 @@ def main():
                 kw_hat = dict(var_penalty=1.0, log_var_target=lvt)
                 for _ in range(3):
                     st_hat(gp.theta_, 2.5, **kw_hat)
-                ctx.timing(True)
-                ctx.timing_reset()
                 t0 = time.perf_counter()
                 for _ in range(args.steps):
                     st_hat(gp.theta_, 2.5, **kw_hat)
                 t_hat = time.perf_counter() - t0
+                ctx.timing(True)
+                ctx.timing_reset()
+                for _ in range(args.steps):
+                    st_hat(gp.theta_, 2.5, **kw_hat)
                 ctx.timing(False)
                 ms_hat, cnt_hat = ctx.timing_read("vecchia_pass1")
                 out["at_theta_hat"] = {"evals_per_s": args.steps / t_hat,

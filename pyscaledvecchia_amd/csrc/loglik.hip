@@ -83,7 +83,7 @@ struct LLArgs {
   const int* chunk;
   const int* level;
   int* fail;
-  const double* inv_lam;
+  double inv_lam[16];   // 1 / range per input (kernel argument, no device copy)
   double* cache;
   double* partial;
   int64_t nblk;
@@ -1968,7 +1968,8 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_tiles(BatchArgs B,
   a.chunk = B.chunk;
   a.level = B.level;
   a.fail = B.fail;
-  a.inv_lam = B.par + i * 16 + 2;
+#pragma unroll
+  for (int l = 0; l < 14; ++l) a.inv_lam[l] = B.par[i * 16 + 2 + l];
   a.cache = B.cache;
   a.partial = nullptr;
   a.nblk = D.blk0 + D.nblk;
@@ -2571,16 +2572,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   const int d = vs.d, q = vs.q;
   hipStream_t s = ctx->stream;
   // parameters
-  double hpar[32];
-  for (int l = 0; l < d; ++l) hpar[l] = 1.0 / std::exp(theta[1 + l]);
-  double* inv_lam = ctx->arena.get_t<double>("v.invlam", 32);
-  SVH_HIP(hipMemcpyAsync(inv_lam, hpar, d * sizeof(double), hipMemcpyHostToDevice, s));
 
   const int nchunk = need_grad ? (est_nu ? vs.nchunk_grad_nu : vs.nchunk_grad) : vs.nchunk_val;
-  int* level = ctx->arena.get_t<int>("v.level", nchunk);
-  int* fail = ctx->arena.get_t<int>("v.fail", nchunk);
-  int_fill(ctx, level, 0, nchunk);
-  int_fill(ctx, fail, 0, nchunk);
+  // jitter levels and failure flags of the chunks, cleared by one launch
+  int* level = ctx->arena.get_t<int>("v.levfail", 2 * (int64_t)nchunk);
+  int* fail = level + nchunk;
+  int_fill(ctx, level, 0, 2 * (int64_t)nchunk);
 
   const int KP = vs.kp;
   const int NR = need_grad ? P + 1 + q : 1 + q;
@@ -2649,7 +2646,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   a.chunk = (const int*)ctx->arena.get(need_grad ? (est_nu ? "v.cgn" : "v.cg") : "v.cv", 0);
   a.level = level;
   a.fail = fail;
-  a.inv_lam = inv_lam;
+  for (int l = 0; l < 16; ++l) a.inv_lam[l] = l < d ? 1.0 / std::exp(theta[1 + l]) : 0.0;
   a.cache = ctx->arena.get_t<double>("v.cache", (size_t)vs.nblk * NC);
   a.partial = ctx->arena.get_t<double>("v.partial", (size_t)Gmax * NACC + 1);
   a.nblk = vs.nblk;
