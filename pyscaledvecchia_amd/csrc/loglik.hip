@@ -1291,7 +1291,6 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   double* xs = hb + 32;                               // coordinates of this half's block
   double* gt = lds + 2 * rows_half_doubles(DM);       // packed lower g(r), both halves
   double* gh = gt + h * kGTri;                        // this half's g
-  double* Mt = gt;                                    // 24 x 33 product operands (after g)
   double* misc = gt + 2 * kGTri;                      // per half: [8h] fail, [8h+1..2] pivot
   const int64_t b = b0 + h;   // first block of the pair: b0
   const bool bvalid = b < a.nblk;
@@ -1441,6 +1440,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   if (r == K - 1) {
     misc[8 * h + 1] = pr;
     misc[8 * h + 2] = dinv;
+    misc[8 * h + 3] = 0.5 * log(pr);   // log L_KK of the block record
   }
   if (r == 31) misc[8 * h] = fail ? 1.0 : 0.0;   // the second 16-lane row saw every pivot
   wave_sync();
@@ -1547,6 +1547,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     if (bh >= a.nblk) break;
     const int Kh = a.kcnt[bh] + 1, km = Kh - 1;
     const int chh = a.chunk[bh];
+    double* Mt = gt;   // 24 x 33 product operands (g is consumed)
     wave_sync();
     if (h == hh) {
       if (grad) {
@@ -1575,7 +1576,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       else if (e <= q)
         v = Mt[(cyl + e) * 33 + km];
       else if (e == q + 1)
-        v = 0.5 * log(misc[8 * hh + 1]);
+        v = misc[8 * hh + 3];
       else
         v = Mt[(e - q - 2) * 33 + km];
       rec[e] = v;
@@ -2608,10 +2609,11 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   const int64_t npair = (vs.nblk + 1) / 2;
   if (use_rows) {
     // kind 2: rows kernel at 2 waves per SIMD, kind 3: at 3 waves per SIMD
-    for (int f : {4, 16}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
+    // (the default: 24 workgroups per CU, measured fastest at C2 size)
+    for (int f : {4, 8, 16}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
+    cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)12 * ncu)});
     dflt = (int)cands.size();
-    cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)8 * ncu)});
-    for (int f : {12, 24}) cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
+    cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)24 * ncu)});
   }
   if (use_split) {
     for (int f : {8, 32}) cands.push_back({1, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
