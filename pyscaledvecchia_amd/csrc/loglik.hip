@@ -1654,9 +1654,9 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
     part[X0 + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
 }
 
-// occ: target waves per SIMD (2: 256 VGPRs, 3: 168 VGPRs with some spilling)
+// two waves per SIMD (at most 256 registers; the kernel needs about 211)
 template <int NU, int DM>
-void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed, int occ) {
+void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed) {
   if constexpr (NU == 2 && DM == 8) {   // phase clocks: FP64, nu = 5/2, d in 5..8
     if (a.prof) {
       hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 2, true>), dim3(G), dim3(kWave), lds, st,
@@ -1664,28 +1664,21 @@ void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool 
       return;
     }
   }
-  if (occ == 3) {
-    if (mixed)
-      hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 3>), dim3(G), dim3(kWave), lds, st, a);
-    else
-      hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 3>), dim3(G), dim3(kWave), lds, st, a);
-  } else {
-    if (mixed)
-      hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
-    else
-      hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
-  }
+  if (mixed)
+    hipLaunchKernelGGL((ll_pass1_rows<NU, true, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
+  else
+    hipLaunchKernelGGL((ll_pass1_rows<NU, false, DM, 2>), dim3(G), dim3(kWave), lds, st, a);
 }
 
 template <int NU>
-void launch_rows_nu(hipStream_t st, int G, int dm, const LLArgs& a, bool mixed, int occ) {
+void launch_rows_nu(hipStream_t st, int G, int dm, const LLArgs& a, bool mixed) {
   const size_t lds = (size_t)rows_lds_doubles(dm) * sizeof(double);
   if (dm == 4)
-    launch_rows_nu_dm<NU, 4>(st, G, lds, a, mixed, occ);
+    launch_rows_nu_dm<NU, 4>(st, G, lds, a, mixed);
   else if (dm == 8)
-    launch_rows_nu_dm<NU, 8>(st, G, lds, a, mixed, occ);
+    launch_rows_nu_dm<NU, 8>(st, G, lds, a, mixed);
   else
-    launch_rows_nu_dm<NU, 10>(st, G, lds, a, mixed, occ);
+    launch_rows_nu_dm<NU, 10>(st, G, lds, a, mixed);
 }
 
 // ---------------------------------------------------------------- reduce 1
@@ -2608,12 +2601,13 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   int dflt = 0;
   const int64_t npair = (vs.nblk + 1) / 2;
   if (use_rows) {
-    // kind 2: rows kernel at 2 waves per SIMD, kind 3: at 3 waves per SIMD
-    // (the default: 24 workgroups per CU, measured fastest at C2 size)
-    for (int f : {4, 8, 16}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
-    cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)12 * ncu)});
+    // kind 2: rows kernel at 2 waves per SIMD (211 registers, no spills; the
+    // 3-wave build spills about 60 registers, and its scratch traffic, up to
+    // 0.7 GB per launch at C2, buys no measurable speed: within 2 %),
+    // default 16 workgroups per CU
+    for (int f : {4, 8}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
     dflt = (int)cands.size();
-    cands.push_back({3, (int)std::min<int64_t>(npair, (int64_t)24 * ncu)});
+    for (int f : {16, 24}) cands.push_back({2, (int)std::min<int64_t>(npair, (int64_t)f * ncu)});
   }
   if (use_split) {
     for (int f : {8, 32}) cands.push_back({1, (int)std::min<int64_t>(vs.nblk, (int64_t)f * ncu)});
@@ -2693,9 +2687,9 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
       if (c.kind >= 2) {
         const int dm = rows_dm(L.a->d, L.a->q);
         switch (L.nu_code) {
-          case 0: launch_rows_nu<0>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
-          case 1: launch_rows_nu<1>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
-          default: launch_rows_nu<2>(L.s, c.G, dm, *L.a, L.mixed, c.kind); break;
+          case 0: launch_rows_nu<0>(L.s, c.G, dm, *L.a, L.mixed); break;
+          case 1: launch_rows_nu<1>(L.s, c.G, dm, *L.a, L.mixed); break;
+          default: launch_rows_nu<2>(L.s, c.G, dm, *L.a, L.mixed); break;
         }
       } else if (c.kind == 1) {
         switch (L.nu_code) {
