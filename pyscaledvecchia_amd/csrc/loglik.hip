@@ -1104,6 +1104,24 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// The kernel's argument block (its only argument, at offset 0 of the kernarg
+// segment) through an address the compiler cannot see to be loop-invariant:
+// fields are loaded from the segment by scalar loads where they are used,
+// instead of held in scalar registers across the loop over block pairs
+// (which spilled them to VGPR lanes and read every 16-register tuple back
+// with readlanes at each use).
+template <class T>
+__device__ __forceinline__ const T& kernarg_fresh() {
+  typedef const __attribute__((address_space(4))) char* kptr;
+  kptr p = (kptr)__builtin_amdgcn_kernarg_segment_ptr();
+#ifdef __OPTIMIZE__
+  // (unoptimised builds keep locals in memory, where no scalar register
+  // operand exists; the hint matters only to the optimised register plan)
+  asm volatile("" : "+s"(p));
+#endif
+  return *(const T*)(const __attribute__((address_space(4))) T*)p;
+}
+
 // v of lane N of this lane's 16-lane row (compiler-managed v_mov_b64 DPP)
 template <int N>
 __device__ __forceinline__ double row_bc(double v) {
@@ -1370,7 +1388,9 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const bool real = i < Kb;
         v = real ? v : 0.0;
         gs = real ? gs : 0.0;
-        if (t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = v;
+        // t = 16 (u = 7, upper half of the wave) is stored by rows 0..15
+        // only; u < 7 gives t <= 15 and needs no test (no branch)
+        if (u < 7 || t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = v;
         gk[8 * hh + u] = gs;
         __builtin_amdgcn_sched_barrier(0);   // keep the iterations apart (register pressure)
       }
@@ -1423,7 +1443,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const int t = 8 * e + 1 + u;
         const int c = (rr + t) & 31;
         const int i = rr > c ? rr : c, j = rr > c ? c : rr;
-        if (t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = gk[8 * hh + u];
+        if (u < 7 || t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = gk[8 * hh + u];
       }
   }
   SVH_PHASE(1)
@@ -1643,7 +1663,7 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
   const int64_t npair = (a.nblk + 1) / 2;
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t pp = blockIdx.x; pp < npair; pp += gridDim.x)
-    rows_pair<NU, MIX, DM, PROF>(a, 2 * pp, lds, accF, accX, ph);
+    rows_pair<NU, MIX, DM, PROF>(kernarg_fresh<LLArgs>(), 2 * pp, lds, accF, accX, ph);
   if (PROF && lane == 0)
     for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   const int NACC = grad ? P * P + q * q + q : q * q + q;

@@ -106,19 +106,24 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 //   mm_select (one workgroup): V = the largest published "next" element over
 //             all slices is an upper bound (in that order) on every point that
 //             is not a published candidate.  Candidates above V form a pool,
-//             kept sorted by (dist desc, index asc).  A step accepts the
-//             longest prefix c_1..c_L of the sorted pool (at most kWin
-//             entries) in which no c_i is lowered by an earlier c_j (squared
-//             distance below c_i's own): c_1 is the serial algorithm's next
-//             pick, and by induction so is each c_i -- its distance is
-//             unchanged by c_1..c_{i-1}, every later pool entry was at or below
-//             it and distances only fall, and every point outside the pool is
-//             at or below V.  The step then lowers the rest of the pool with
-//             the L accepted points; entries falling to V or below leave the
-//             pool, the others keep their order (a stable compaction), and
-//             only when some entry was lowered but stays above V is the pool
-//             sorted again.  The round ends when the pool is empty.  On
-//             well-spread inputs most steps accept the whole window.
+//             kept sorted by (dist desc, index asc).  A step accepts every
+//             pool entry that no entry above it would lower (squared distance
+//             below the entry's own).  Such an entry c is a pick of the
+//             serial algorithm with dist(c) as its distance: the points the
+//             serial algorithm picks before c have keys above c's at their
+//             pick; keys only fall, so they are pool entries above c now (or
+//             accepted earlier in the round and already applied to c, below),
+//             none of which lowers c; points outside the pool are at or below
+//             V.  Blocked entries stay.  A remaining entry is lowered by the
+//             round's accepted points that precede it in the serial order --
+//             those with (key, index) above its own -- to a fixpoint, since a
+//             lowered entry comes to follow more of them; entries falling to V
+//             or below leave the pool, the others keep their order (a stable
+//             compaction), sorted again when one was lowered.  The round ends
+//             when the pool is empty; its picks are the accepted points in
+//             (key desc, index asc) order, the serial order (every later pick
+//             is at or below V).  On well-spread inputs most rounds take one
+//             step.
 // Coordinates are held in DM >= d registers per point with the extra
 // coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
 // the squared distances are bit-identical to the d-term sums.
@@ -134,7 +139,6 @@ constexpr int kPool = 256;            // pool positions = select threads
 constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
 constexpr int kMMD = 12;              // max d of the maximin kernels
-constexpr int kWin = 64;              // prefix window of one selection step (one wave)
 
 struct MMState {
   int count;  // points placed in `order`
@@ -448,6 +452,103 @@ __device__ __forceinline__ void pool_sort(double& k, int& i, int& s, double* bk,
   pool_sort_size<NT, 2>(k, i, s, bk, bi, bs);
 }
 
+typedef float mm_v4f __attribute__((ext_vector_type(4)));
+
+// Margin of the Gram-form screen below: with the coordinates rounded to
+// float32 (norms <= m2d), D = |a|^2 + |b|^2 - 2 a.b from float32 norms
+// (sequential fmaf, <= 12 terms) and float32 matrix-core dot products (at
+// most 12 products accumulated) differs from the squared distance of the
+// rounded coordinates by at most about 62 * 2^-24 * m2d; the rounding of
+// the coordinates themselves is covered by screen_threshold.  2^-15 m2d is
+// eight times that.
+__device__ __forceinline__ float gram_threshold(double s, double m2d) {
+  const double T = s * (1.0 + 0x1p-14) + 0x1p-16 * sqrt(m2d * s) + 0x1p-38 * m2d + 0x1p-15 * m2d;
+  return (T > 1e-30 && T < 1e30) ? (float)T : INFINITY;
+}
+
+// First-step conflicts of a selection round on the matrix cores: D(i, j) for
+// the N pool entries by position (rows i, columns j < i) in 16 x 16 tiles of
+// v_mfma_f32_16x16x4f32 (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and holds D[4 (l / 16) + v][l % 16], v = 0..3); pairs with D below row i's
+// Gram threshold are tested exactly and block row i when they lower it.  The
+// 16 tile rows go to the four waves as {w, 7 - w, 8 + w, 15 - w} (34 tiles
+// each at N = 256).
+template <int DM>
+__device__ __forceinline__ void mm_conflicts_mfma(int N, const float* pf, const float* pn,
+                                                  const float* pt, const double* pc,
+                                                  const int* cs, const double* ps2, int* blk,
+                                                  int lane, int wid) {
+  constexpr int KS = (DM + 3) / 4;
+  const int nI = (N + 15) >> 4;
+  const int i16 = lane & 15, kq = lane >> 4;
+  // loads are unconditional (indices clamped into the live entries) and the
+  // values selected afterwards: no branches around LDS reads
+  auto coord = [&](int p, int k) {
+    const float v = pf[min(p, N - 1) * DM + min(k, DM - 1)];
+    return (p < N) & (k < DM) ? v : 0.0f;
+  };
+#pragma unroll 1
+  for (int t = 0; t < 4; ++t) {
+    const int I = t == 0 ? wid : (t == 1 ? 7 - wid : (t == 2 ? 8 + wid : 15 - wid));
+    if (I >= nI) continue;
+    float a[KS];
+#pragma unroll
+    for (int sl = 0; sl < KS; ++sl) a[sl] = coord(16 * I + i16, 4 * sl + kq);
+    float rn[4], rt[4];
+    int rr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      rr[v] = 16 * I + 4 * kq + v;
+      const int rc = min(rr[v], N - 1);
+      rn[v] = pn[rc];
+      rt[v] = rr[v] < N ? pt[rc] : -INFINITY;
+    }
+    // four column tiles at a time (independent loads and products in flight)
+#pragma unroll 1
+    for (int J0 = 0; J0 <= I; J0 += 4) {
+      mm_v4f acc[4];
+      float cn[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int cj = 16 * (J0 + u) + i16;
+        float bv[KS];
+#pragma unroll
+        for (int sl = 0; sl < KS; ++sl) bv[sl] = coord(cj, 4 * sl + kq);
+        cn[u] = pn[min(cj, N - 1)];
+        acc[u] = mm_v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int sl = 0; sl < KS; ++sl)
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sl], bv[sl], acc[u], 0, 0, 0);
+      }
+      unsigned fl = 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int cj = 16 * (J0 + u) + i16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float D = fmaf(-2.0f, acc[u][v], rn[v] + cn[u]);
+          // column above the row (cj < rr <= N - 1 for live rows), below the threshold
+          const bool f = (cj < rr[v]) & (D < rt[v]);
+          fl |= (unsigned)f << (4 * u + v);
+        }
+      }
+      while (fl) {
+        const int bit = __builtin_ctz(fl);
+        fl &= fl - 1u;
+        const int u = bit >> 2, v = bit & 3;
+        const int cj = 16 * (J0 + u) + i16;
+        const int r = 16 * I + 4 * kq + v;
+        const double* xa = pc + cs[r] * DM;
+        const double* xb = pc + cs[cj] * DM;
+        double e = 0.0;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) e = sq_step(e, xa[l], xb[l]);
+        if (e < ps2[r]) blk[r] = 1;
+      }
+    }
+  }
+}
+
 template <int DM, int TOP>
 __global__ __launch_bounds__(kPool) void mm_select_kernel(
     const MMDesc* __restrict__ desc, const double* __restrict__ XTall,
@@ -458,7 +559,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   // prof (profiling runs only): shader clocks of thread 0 per phase -- set-up,
   // window checks, compactions, re-sorts, pool distance updates -- and the
   // number of steps
-  unsigned long long ph[7] = {0, 0, 0, 0, 0, 0, 0}, steps = 0;
+  unsigned long long ph[7] = {0, 0, 0, 0, 0, 0, 0}, steps = 0, passes = 0, rsum = 0, lsteps = 0;
   unsigned long long ph_t = prof ? __builtin_amdgcn_s_memtime() : 0;
 #define MM_PHASE(k)                                            \
   if (prof) {                                                  \
@@ -467,7 +568,7 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     ph_t = t_;                                                 \
   }
   constexpr int NT = kPool, NW = NT / kWave;
-  static_assert(kWin == kWave && 4 * kWin <= NT, "window layout");
+  static_assert(kAccCap == NT, "accepted points of a round: at most the pool");
   const MMDesc D = desc[blockIdx.x];
   const int n = D.n, d = D.d, nslice = D.nslice;
   const double* XT = XTall + D.xoff;
@@ -489,14 +590,23 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   __shared__ int qi[2][NT], qs[2][NT];
   __shared__ double rk[NW];
   __shared__ int ri[NW], wcnt[NW];
-  __shared__ unsigned long long wlow[NW], wlwr[NW];
-  __shared__ int stepL;
-  __shared__ double wc[kWin * DM], ws2[kWin];
-  __shared__ float wcf[kWin * DM];
+  __shared__ int wcnt2[NW];
+  __shared__ unsigned long long wlwr[NW];
+  // steps: float32 pool coordinates by slot and live entries' by position,
+  // squared distances by position, blocked flags, the lowering passes'
+  // squared distances (as 64-bit patterns: non-negative doubles order alike)
+  __shared__ float pcf[NT * DM], pf[NT * DM];
+  __shared__ double ps2[NT];
+  __shared__ unsigned long long nsv[NT];
+  __shared__ int blk[NT];
+  __shared__ float pn[NT], pt[NT];   // first step on the matrix cores: norms, thresholds
+  __shared__ double rmin[NW];
+  __shared__ double pkp[NT];   // lowering passes: each entry's key at the previous pass
   // this round's accepted points (index, pool slot), written out at the end
   // of the round: global stores inside the steps would make every barrier
   // wait for their completion
   __shared__ int acc_idx[kAccCap], acc_slot[kAccCap];
+  __shared__ double acc_key[kAccCap];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
   const int count0 = st->count;
   const double Mabs = __longlong_as_double((long long)st->mabs);
@@ -657,135 +767,281 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
   }
   __syncthreads();
   int cur = 0, nlive = nsurv, nacc = 0;
-  const int room = n - count0;
-  const int cap = room < kAccCap ? room : kAccCap;
+  // float32 copies of the pool coordinates by slot (the steps' screens)
+  for (int e = tid; e < NT * DM; e += NT) pcf[e] = (float)pc[e];
+  // The first step's conflicts go to the matrix cores (Gram form) unless its
+  // margin 2^-15 m2d is large against the pool's squared distances (inputs
+  // far from the origin against their spacing) or the scale is extreme;
+  // then, and for the later (small) steps, the direct-difference screen.
+  {
+    double m = tid < nsurv ? pss[tid] : INFINITY;
+    for (int off = 32; off >= 1; off >>= 1) m = fmin(m, __shfl_xor(m, off, kWave));
+    if (lane == 0) rmin[wid] = m;
+  }
+  __syncthreads();
+  double smin = rmin[0];
+#pragma unroll
+  for (int w2 = 1; w2 < NW; ++w2) smin = fmin(smin, rmin[w2]);
+  const bool use_mfma = nsurv > 64 && m2d > 1e-20 && m2d < 1e30 && 0x1p-15 * m2d <= 0.25 * smin;
   MM_PHASE(0)
-  while (nacc < cap && nlive > 0) {
+  while (nlive > 0) {
     ++steps;
     const double* ck = qk[cur];
     const int* cidx = qi[cur];
     const int* cs = qs[cur];
-    // the window's coordinates and squared distances, contiguous by position
-    const int nw = nlive < kWin ? nlive : kWin;
-    for (int e = tid; e < kWin * DM; e += NT) {
-      const int w = e / DM;
-      const double v = w < nw ? pc[cs[w] * DM + (e - w * DM)] : 0.0;
-      wc[e] = v;
-      wcf[e] = (float)v;
+    // the live entries by position: float32 coordinates, squared distances
+    for (int e = tid; e < nlive * DM; e += NT) {
+      const int p = e / DM;
+      pf[e] = pcf[cs[p] * DM + (e - p * DM)];
     }
-    if (tid < kWin) ws2[tid] = tid < nw ? pss[cs[tid]] : 0.0;
+    const bool mfma_step = use_mfma && steps == 1;
+    if (tid < nlive) {
+      const double sv = pss[cs[tid]];
+      ps2[tid] = sv;
+      blk[tid] = 0;
+      if (mfma_step) {
+        float nr = 0.0f;
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          const float x = pcf[cs[tid] * DM + l];
+          nr = fmaf(x, x, nr);
+        }
+        pn[tid] = nr;
+        pt[tid] = gram_threshold(sv, m2d);
+      }
+    }
     __syncthreads();
-    // window conflicts: is c_w lowered by an earlier c_j?  Four threads per
-    // window entry (consecutive lanes), each taking every fourth j.
-    {
-      const int w = tid >> 2, wsub = tid & 3;
-      bool low = false;
-      if (w > 0 && w < nw) {
-        const double s_w = ws2[w];
-        double xw[DM];
+    // Conflicts: entry p is blocked when an entry above it (position j < p)
+    // would lower it (canonical squared distance below p's own).  The
+    // N(N-1)/2 pairs are split evenly over the threads in row-major order;
+    // float32 screen first, the exact distance for flagged pairs only.
+    if (mfma_step) {
+      mm_conflicts_mfma<DM>(nlive, pf, pn, pt, pc, cs, ps2, blk, lane, wid);
+    } else {
+      const int N = nlive;
+      const int T = N * (N - 1) / 2;
+      const int per = (T + NT - 1) / NT;
+      int q = tid * per;
+      const int qe = q + per < T ? q + per : T;
+      if (q < qe) {
+        // row p holds the pairs q in [p(p-1)/2, p(p+1)/2)
+        int p = (int)(0.5f * (1.0f + sqrtf(1.0f + 8.0f * (float)q)));
+        while (p * (p - 1) / 2 > q) --p;
+        while (p * (p + 1) / 2 <= q) ++p;
+        int j = q - p * (p - 1) / 2;
+        float xp[DM];
 #pragma unroll
-        for (int l = 0; l < DM; ++l) xw[l] = wc[w * DM + l];
-        for (int j = wsub; j < w; j += 4) {
-          double t = 0.0;
+        for (int l = 0; l < DM; ++l) xp[l] = pf[p * DM + l];
+        double sp = ps2[p];
+        float thr = screen_threshold(sp, m2d);
+        while (q < qe) {
+          bool low = false;
+          int adv = 1;
+          // exact test of flagged pair (p, jj): is p lowered by jj?
+          auto exact_low = [&](int jj) {
+            const double* xa = pc + cs[p] * DM;
+            const double* xb = pc + cs[jj] * DM;
+            double t = 0.0;
 #pragma unroll
-          for (int l = 0; l < DM; ++l) t = sq_step(t, xw[l], wc[j * DM + l]);
-          low = low || (t < s_w);
+            for (int l = 0; l < DM; ++l) t = sq_step(t, xa[l], xb[l]);
+            return t < sp;
+          };
+          if (q + 4 <= qe && j + 4 <= p) {
+            // four pairs of row p at once: independent screen chains
+            float t4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int l = 0; l < DM; ++l)
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const float df = xp[l] - pf[(j + u) * DM + l];
+                t4[u] = fmaf(df, df, t4[u]);
+              }
+            unsigned fl = 0u;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) fl |= (t4[u] < thr) ? (1u << u) : 0u;
+            adv = 4;
+            while (fl && !low) {
+              const int u = __builtin_ctz(fl);
+              fl &= fl - 1u;
+              low = exact_low(j + u);
+            }
+          } else if (screen_sq<DM>(xp, pf + j * DM) < thr) {
+            low = exact_low(j);
+          }
+          if (low) {
+            blk[p] = 1;
+            adv = p - j;   // the rest of row p is not needed
+          }
+          q += adv;
+          j += adv;
+          if (j >= p && q < qe) {   // next row (exists: q < T)
+            ++p;
+            j = 0;
+#pragma unroll
+            for (int l = 0; l < DM; ++l) xp[l] = pf[p * DM + l];
+            sp = ps2[p];
+            thr = screen_threshold(sp, m2d);
+          }
         }
       }
-      const unsigned long long lb = __ballot(low);
-      if (lane == 0) wlow[wid] = lb;
     }
     __syncthreads();
-    if (wid == 0) {
-      // window entry `lane`: bad when lowered or outside the pool
-      const unsigned nib = (unsigned)(wlow[lane >> 4] >> (4 * (lane & 15))) & 0xFu;
-      const bool bad = (lane > 0 && nib != 0u) || lane >= nw;
-      const unsigned long long bb = __ballot(bad);
-      int L = bb ? (int)__ffsll((long long)bb) - 1 : kWin;
-      if (L > cap - nacc) L = cap - nacc;
-      if (lane == 0) stepL = L;
-    }
-    __syncthreads();
-    const int L = stepL;   // >= 1: position 0 is in the pool
     MM_PHASE(1)
-    if (tid < L) {
-      acc_idx[nacc + tid] = cidx[tid];
-      acc_slot[nacc + tid] = cs[tid];
-    }
-    // lower the rest of the pool (position tid) with the accepted points.
-    // Lane j of every wave holds accepted point j; the loop broadcasts its
-    // coordinates with readlane (scalar operands) instead of LDS reads.
-    float axf[DM];
+    // every unblocked entry is accepted; blocked entries stay, in order
+    {
+      const bool live = tid < nlive;
+      const bool take = live && blk[tid] == 0;
+      const bool stay = live && !take;
+      const unsigned long long mt = __ballot(take), my = __ballot(stay);
+      if (lane == 0) {
+        wcnt[wid] = __popcll(mt);
+        wcnt2[wid] = __popcll(my);
+      }
+      __syncthreads();
+      int ot = 0, oy = 0, tt = 0, ty = 0;
 #pragma unroll
-    for (int l = 0; l < DM; ++l) axf[l] = lane < L ? wcf[lane * DM + l] : 0.0f;
+      for (int w2 = 0; w2 < NW; ++w2) {
+        ot += (w2 < wid) ? wcnt[w2] : 0;
+        oy += (w2 < wid) ? wcnt2[w2] : 0;
+        tt += wcnt[w2];
+        ty += wcnt2[w2];
+      }
+      const unsigned long long below = (1ull << lane) - 1ull;
+      if (take) {
+        const int k = nacc + ot + __popcll(mt & below);
+        acc_idx[k] = cidx[tid];
+        acc_slot[k] = cs[tid];
+        acc_key[k] = ck[tid];
+      }
+      if (stay) {
+        const int np = oy + __popcll(my & below);
+        qk[1 - cur][np] = ck[tid];
+        qi[1 - cur][np] = cidx[tid];
+        qs[1 - cur][np] = cs[tid];
+      }
+      nacc += tt;
+      nlive = ty;
+      cur = 1 - cur;
+      __syncthreads();
+    }
+    MM_PHASE(2)
+    if (nlive == 0) break;
+    // The remaining entries are lowered by the round's accepted points that
+    // precede them in the serial order: (key, index) above the entry's own.
+    // Keys only fall, so an entry that is lowered may come to follow more
+    // accepted points: passes until no entry changes, each examining only
+    // the points that precede the entry at its current key and did not at
+    // the key of the previous pass (kp, +inf at first).
+    ++lsteps;
+    rsum += nlive;
+    if (tid < nlive) {
+      const double sv = pss[qs[cur][tid]];
+      ps2[tid] = sv;
+      pkp[tid] = INFINITY;
+      nsv[tid] = (unsigned long long)__double_as_longlong(sv);
+    }
+    __syncthreads();
+    for (;;) {
+      ++passes;
+      const int T = nlive * nacc;
+      const int per = (T + NT - 1) / NT;
+      int q = tid * per;
+      const int qe = q + per < T ? q + per : T;
+      if (q < qe) {
+        int c = q / nacc, a = q - c * nacc;
+        int slc = qs[cur][c], ic = qi[cur][c];
+        double sc = ps2[c];
+        double kc = sqrt(sc), kp = pkp[c];
+        float thr = screen_threshold(sc, m2d);
+        float xc[DM];
+#pragma unroll
+        for (int l = 0; l < DM; ++l) xc[l] = pcf[slc * DM + l];
+        while (q < qe) {
+          const int nu = (q + 4 <= qe && a + 4 <= nacc) ? 4 : 1;
+          // four accepted points (one at a row's end): loads, screens and
+          // precedence tests without branches, exact distances for the
+          // flagged ones
+          int sa4[4], ia4[4];
+          double ka4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int au = a + (u < nu ? u : 0);
+            sa4[u] = acc_slot[au];
+            ia4[u] = acc_idx[au];
+            ka4[u] = acc_key[au];
+          }
+          float t4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int l = 0; l < DM; ++l)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const float df = xc[l] - pcf[sa4[u] * DM + l];
+              t4[u] = fmaf(df, df, t4[u]);
+            }
+          unsigned el = 0u;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool now = (ka4[u] > kc) | ((ka4[u] == kc) & (ia4[u] < ic));
+            const bool before = (ka4[u] > kp) | ((ka4[u] == kp) & (ia4[u] < ic));
+            el |= (unsigned)((u < nu) & now & !before & (t4[u] < thr)) << u;
+          }
+          while (el) {
+            const int u = __builtin_ctz(el);
+            el &= el - 1u;
+            double t = 0.0;
+#pragma unroll
+            for (int l = 0; l < DM; ++l) t = sq_step(t, pc[slc * DM + l], pc[sa4[u] * DM + l]);
+            if (t < sc) atomicMin(&nsv[c], (unsigned long long)__double_as_longlong(t));
+          }
+          q += nu;
+          a += nu;
+          if (a == nacc && q < qe) {
+            a = 0;
+            ++c;
+            slc = qs[cur][c];
+            ic = qi[cur][c];
+            sc = ps2[c];
+            kc = sqrt(sc);
+            kp = pkp[c];
+            thr = screen_threshold(sc, m2d);
+#pragma unroll
+            for (int l = 0; l < DM; ++l) xc[l] = pcf[slc * DM + l];
+          }
+        }
+      }
+      __syncthreads();
+      bool changed = false;
+      if (tid < nlive) {
+        const double ns = __longlong_as_double((long long)nsv[tid]);
+        pkp[tid] = sqrt(ps2[tid]);
+        if (ns < ps2[tid]) {
+          ps2[tid] = ns;
+          changed = true;
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+    MM_PHASE(4)
+    // entries falling to V or below leave the pool; the others keep their
+    // order (a stable compaction), sorted again when one was lowered
     double key = -3.0;
     int idx = INT_MAX, slot = 0;
     bool lowered = false;
-    if (tid >= L && tid < nlive) {
-      slot = cs[tid];
-      key = ck[tid];
-      idx = cidx[tid];
-      const double s = pss[slot];
-      double x[DM];
-      float xf[DM];
-#pragma unroll
-      for (int l = 0; l < DM; ++l) {
-        x[l] = pc[slot * DM + l];
-        xf[l] = (float)x[l];
-      }
-      // float32 screen against the L accepted points (lane j of every wave
-      // holds accepted point j: readlane gives scalar operands; four
-      // independent chains per iteration), then the exact canonical
-      // distance for the flagged points only
-      const float thr = screen_threshold(s, m2d);
-      unsigned long long flag = 0ull;
-      int j = 0;
-      for (; j + 4 <= L; j += 4) {
-        float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int l = 0; l < DM; ++l) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const float av = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(axf[l]), j + u));
-            const float df = xf[l] - av;
-            t[u] = fmaf(df, df, t[u]);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (t[u] < thr) flag |= 1ull << (j + u);
-      }
-      for (; j < L; ++j) {
-        float t = 0.0f;
-#pragma unroll
-        for (int l = 0; l < DM; ++l) {
-          const float av = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(axf[l]), j));
-          const float df = xf[l] - av;
-          t = fmaf(df, df, t);
-        }
-        if (t < thr) flag |= 1ull << j;
-      }
-      double smin = s;
-      while (flag) {
-        const int jj = __builtin_ctzll(flag);
-        flag &= flag - 1ull;
-        double t = 0.0;
-#pragma unroll
-        for (int l = 0; l < DM; ++l) t = sq_step(t, x[l], wc[jj * DM + l]);
-        smin = t < smin ? t : smin;
-      }
-      if (smin < s) {
-        pss[slot] = smin;
-        key = sqrt(smin);
+    if (tid < nlive) {
+      slot = qs[cur][tid];
+      idx = qi[cur][tid];
+      key = qk[cur][tid];
+      const double sv = ps2[tid];
+      if (sv < pss[slot]) {
+        pss[slot] = sv;
+        key = sqrt(sv);
         if (mm_greater(key, idx, Vk, Vi))
           lowered = true;
         else
-          key = -3.0;   // at or below V: leaves the pool
+          key = -3.0;
       }
     }
-    MM_PHASE(4)
     const bool live = key >= 0.0;
-    // stable compaction of the remaining entries into the other buffer
     const unsigned long long lm = __ballot(live), lw = __ballot(lowered);
     if (lane == 0) {
       wcnt[wid] = __popcll(lm);
@@ -806,7 +1062,6 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
       qi[1 - cur][np] = idx;
       qs[1 - cur][np] = slot;
     }
-    nacc += L;
     nlive = tot;
     cur = 1 - cur;
     __syncthreads();
@@ -825,13 +1080,21 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     }
   }
   __syncthreads();
-  for (int k = tid; k < nacc; k += NT) {
-    const int idx = acc_idx[k], sl = acc_slot[k];
-    acc[k] = idx;
-    order[count0 + k] = idx;
+  // the round's points in the serial order: (key desc, index asc) -- every
+  // later pick has a key at or below V, below all of them.  A round of one
+  // step accepted them in pool order, already sorted.
+  {
+    double k = tid < nacc ? acc_key[tid] : -3.0;
+    int i = tid < nacc ? acc_idx[tid] : INT_MAX, sl = tid < nacc ? acc_slot[tid] : 0;
+    if (steps > 1) pool_sort<NT>(k, i, sl, qk[0], qi[0], qs[0]);
+    if (tid < nacc) {
+      acc[tid] = i;
+      order[count0 + tid] = i;
 #pragma unroll
-    for (int l = 0; l < kMMD; ++l) accx[k * kMMD + l] = l < DM ? pc[sl * DM + l] : 0.0;
+      for (int l = 0; l < kMMD; ++l) accx[tid * kMMD + l] = l < DM ? pc[sl * DM + l] : 0.0;
+    }
   }
+  MM_PHASE(3)
   if (tid == 0) {
     st->nacc = nacc;
     st->count = count0 + nacc;
@@ -839,6 +1102,9 @@ __global__ __launch_bounds__(kPool) void mm_select_kernel(
     if (prof) {
       for (int k = 0; k < 7; ++k) atomicAdd(prof + k, ph[k]);
       atomicAdd(prof + 7, steps);
+      atomicAdd(prof + 8, passes);
+      atomicAdd(prof + 9, rsum);
+      atomicAdd(prof + 10, lsteps);
     }
   }
 #undef MM_PHASE
@@ -1342,8 +1608,8 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   MMState* hst = static_cast<MMState*>(ctx->stage.get(nd * sizeof(MMState)));
   unsigned long long* mprof = nullptr;   // per-phase clocks of mm_select (profiling runs)
   if (std::getenv("SVH_MM_PROFILE")) {
-    mprof = ctx->arena.get_t<unsigned long long>("mm.prof", 8);
-    const unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    mprof = ctx->arena.get_t<unsigned long long>("mm.prof", 12);
+    const unsigned long long z[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     SVH_HIP(hipMemcpy(mprof, z, sizeof z, hipMemcpyHostToDevice));
   }
   int batch = 8;
@@ -1391,14 +1657,14 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   ctx->mm_rounds = rounds;
   ctx->mm_fallbacks = fallbacks;
   if (mprof) {
-    unsigned long long h[8];
+    unsigned long long h[12];
     SVH_HIP(hipMemcpy(h, mprof, sizeof h, hipMemcpyDeviceToHost));
     int sorts = 0;
     for (int i = 0; i < nd; ++i) sorts += hst[i].overflow_sorts;
     std::fprintf(stderr, "[svh maximin clocks] rounds %d steps %llu setup: bound %llu survivors "
                  "%llu pool %llu; window %llu distances %llu compaction %llu resort %llu "
-                 "survivor-sorts %d\n", rounds, h[7], h[5], h[6], h[0], h[1], h[4], h[2], h[3],
-                 sorts);
+                 "survivor-sorts %d lowering steps %llu passes %llu entries %llu\n", rounds, h[7], h[5], h[6], h[0], h[1], h[4], h[2], h[3],
+                 sorts, h[10], h[8], h[9]);
   }
 }
 
