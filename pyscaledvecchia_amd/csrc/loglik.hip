@@ -1723,6 +1723,24 @@ __global__ __launch_bounds__(256) void ll_sum_partials(const double* __restrict_
   if (threadIdx.x == 0) tot[e] = red[0];
 }
 
+// Two-level sum of the pass-1 partials (G rows of NACC values): workgroup b
+// adds rows [b*kSumRows, (b+1)*kSumRows) in order, reading whole rows
+// (consecutive threads, consecutive entries); ll_sum_solve adds the level
+// sums in order.  Reading a column per workgroup instead touched every
+// 128-byte line once per entry it holds.
+constexpr int kSumRows = 64;
+__global__ __launch_bounds__(256) void ll_sum_rows(const double* __restrict__ partial, int G,
+                                                   int NACC, double* __restrict__ lvl) {
+  const int g0 = blockIdx.x * kSumRows;
+  const int g1 = g0 + kSumRows < G ? g0 + kSumRows : G;
+  for (int e = threadIdx.x; e < NACC; e += 256) {
+    double s = 0.0;
+#pragma unroll 16
+    for (int g = g0; g < g1; ++g) s += partial[(int64_t)g * NACC + e];
+    lvl[(int64_t)blockIdx.x * NACC + e] = s;
+  }
+}
+
 // Fisher to `out`, beta = solve(XtX + 1e-12 I, Zty) by Gaussian elimination
 // with partial pivoting, parallel over the trailing submatrix in LDS.
 // Fisher entries are stored with row stride `fst` (P for one structure).
@@ -1789,6 +1807,87 @@ __global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ 
                                                      int need_grad, double* __restrict__ beta,
                                                      double* __restrict__ out) {
   ll_solve_beta_body(tot, P, q, need_grad, beta, out, P, P);
+}
+
+// beta = solve(XtX + 1e-12 I, Zty), q <= 32, by one wave (lane i holds row i of the
+// augmented system in LDS): the pivot (first largest |A[i][k]|, i >= k) by a
+// wave arg-max, rows exchanged column-parallel, each row eliminated by its
+// lane with A[i][j] -= (A[i][k] / A[k][k]) * A[k][j], back substitution by
+// columns (x_c formed, then every row above subtracts A[r][c] x_c).  The
+// caller has written the Fisher block of `out`.
+__device__ void ll_solve_beta_wave(const double* __restrict__ tot, int P, int q, int need_grad,
+                                   double* __restrict__ beta, double* __restrict__ out) {
+  __shared__ double A[32][34];
+  const int X0 = need_grad ? P * P : 0, Y0 = X0 + q * q;
+  const int lane = threadIdx.x;   // one wave
+  for (int e = lane; e < q * (q + 1); e += kWave) {
+    const int i = e / (q + 1), j = e - i * (q + 1);
+    A[i][j] = (j < q) ? tot[X0 + i * q + j] + (i == j ? 1e-12 : 0.0) : tot[Y0 + i];
+  }
+  wave_sync();
+  for (int k = 0; k < q; ++k) {
+    double v = (lane >= k && lane < q) ? fabs(A[lane][k]) : -1.0;
+    int p = lane;
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double ov = __shfl_xor(v, off, kWave);
+      const int op = __shfl_xor(p, off, kWave);
+      if (ov > v || (ov == v && op < p)) {
+        v = ov;
+        p = op;
+      }
+    }
+    if (p != k && lane <= q) {
+      const double t = A[k][lane];
+      A[k][lane] = A[p][lane];
+      A[p][lane] = t;
+    }
+    wave_sync();
+    if (lane > k && lane < q) {
+      const double f = A[lane][k] / A[k][k];
+      for (int j = k + 1; j <= q; ++j) A[lane][j] -= f * A[k][j];
+    }
+    wave_sync();
+  }
+  double sr = lane < q ? A[lane][q] : 0.0, x = 0.0;
+  for (int c = q - 1; c >= 0; --c) {
+    const double xc = __shfl(sr, c, kWave) / A[c][c];
+    if (lane == c) x = xc;
+    if (lane < c) sr -= A[lane][c] * xc;
+  }
+  if (lane < q) {
+    beta[lane] = x;
+    out[1 + P + P * P + lane] = x;
+  }
+}
+
+// Second level of the partial sums, then the solve of ll_solve_beta.
+__global__ __launch_bounds__(1024) void ll_sum_solve(const double* __restrict__ lvl, int GS,
+                                                     int NACC, int P, int q, int need_grad,
+                                                     double* __restrict__ beta,
+                                                     double* __restrict__ out) {
+  __shared__ double stot[40 * 40 + 32 * 32 + 32];   // NACC at P <= 40, q <= 32 (set-up limits)
+  __shared__ double part4[256][4];
+  // four threads per entry (every fourth level sum each, loads in flight),
+  // combined in a fixed order
+  const int sub = threadIdx.x & 3;
+  for (int e0 = 0; e0 < NACC; e0 += 256) {
+    const int e = e0 + (threadIdx.x >> 2);
+    double s = 0.0;
+    if (e < NACC) {
+#pragma unroll 8
+      for (int b = sub; b < GS; b += 4) s += lvl[(int64_t)b * NACC + e];
+    }
+    part4[threadIdx.x >> 2][sub] = s;
+    __syncthreads();
+    if (sub == 0 && e < NACC) {
+      const double t = (part4[threadIdx.x >> 2][0] + part4[threadIdx.x >> 2][1]) +
+                       (part4[threadIdx.x >> 2][2] + part4[threadIdx.x >> 2][3]);
+      stot[e] = t;
+      if (need_grad && e < P * P) out[1 + P + e] = t;   // Fisher
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kWave && q > 0) ll_solve_beta_wave(stot, P, q, need_grad, beta, out);
 }
 
 // ---------------------------------------------------------------- pass 2
@@ -2688,6 +2787,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, (int*)a.prof, 0, (int64_t)16);
   }
   double* tot = ctx->arena.get_t<double>("v.tot", NACC + 1);
+  double* lvl = ctx->arena.get_t<double>("v.lvl", (size_t)((Gmax + kSumRows - 1) / kSumRows) * NACC + 1);
   double* dbeta = ctx->arena.get_t<double>("v.beta", 33);
   double* partial2 = ctx->arena.get_t<double>("v.partial2", (size_t)G2 * NV);
   const int NOUT = 1 + P + P * P + q + 1;
@@ -2759,9 +2859,16 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     Launch::run(&lau, 0);
     SVH_HIP(hipGetLastError());
     ctx->timers.end(s);
-    if (NACC > 0)
-      hipLaunchKernelGGL(ll_sum_partials, dim3(NACC), dim3(256), 0, s, a.partial, G, NACC, tot);
-    hipLaunchKernelGGL(ll_solve_beta, dim3(1), dim3(256), 0, s, tot, P, q, need_grad, dbeta, dout);
+    if (NACC > 0) {
+      const int GS = (G + kSumRows - 1) / kSumRows;
+      hipLaunchKernelGGL(ll_sum_rows, dim3(GS), dim3(256), 0, s, a.partial, G, NACC, lvl);
+      SVH_REQUIRE(NACC <= 40 * 40 + 32 * 32 + 32, "evaluation: too many accumulator entries");
+      hipLaunchKernelGGL(ll_sum_solve, dim3(1), dim3(1024), 0, s, lvl, GS, NACC, P, q, need_grad,
+                         dbeta, dout);
+    } else {
+      hipLaunchKernelGGL(ll_solve_beta, dim3(1), dim3(256), 0, s, tot, P, q, need_grad, dbeta,
+                         dout);
+    }
     hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), 0, s, a.cache, vs.nblk, P, q, need_grad, G2L,
                        dbeta, partial2);
     hipLaunchKernelGGL(ll_reduce2, dim3(NV + 1), dim3(256), 0, s, partial2, G2, NV, fail, nchunk,
