@@ -1181,10 +1181,10 @@ __device__ __forceinline__ void rhs_self(T (&V)[NC], T m) {
 // those columns copied to both 16-lane rows.
 template <int C, int G, int J, int NC, class T>
 __device__ __forceinline__ void rhs_from_lo(T (&V)[NC], const T (&W)[4], double (&S)[32],
-                                            bool hi_row) {
+                                            double hm) {
   if constexpr (J < 16) {
     asm volatile("" : "+v"(S[J]));
-    const T m = hi_row ? (T)S[J] : (T)0;
+    const T m = (T)(S[J] * hm);   // hm: 1 on rows 16..31, 0 on rows 0..15
     if constexpr (G == 4) {
       if constexpr (sizeof(T) == 8)
         bf4_src<J>(V[C], V[C + 1], V[C + 2], V[C + 3], W[0], W[1], W[2], W[3], m);
@@ -1196,7 +1196,7 @@ __device__ __forceinline__ void rhs_from_lo(T (&V)[NC], const T (&W)[4], double 
       else
         bf2f_src<J>(V[C], V[C + 1], W[0], W[1], m);
     }
-    rhs_from_lo<C, G, J + 1, NC>(V, W, S, hi_row);
+    rhs_from_lo<C, G, J + 1, NC>(V, W, S, hm);
   }
 }
 __device__ __forceinline__ double lo_row_both(double v) { return from_lo_row(v); }
@@ -1206,20 +1206,20 @@ __device__ __forceinline__ float lo_row_both(float v) {
   return __uint_as_float(sw[0]);
 }
 template <int C, int NC, class T>
-__device__ __forceinline__ void rhs_lo_to_hi_from(T (&V)[NC], double (&S)[32], int r) {
+__device__ __forceinline__ void rhs_lo_to_hi_from(T (&V)[NC], double (&S)[32], double hm) {
   if constexpr (C < NC) {
     constexpr int G = (NC - C >= 4) ? 4 : 2;
     T W[4];
 #pragma unroll
     for (int u = 0; u < G; ++u) W[u] = lo_row_both(V[C + u]);
-    rhs_from_lo<C, G, 0, NC>(V, W, S, r >= 16);
+    rhs_from_lo<C, G, 0, NC>(V, W, S, hm);
     __builtin_amdgcn_sched_barrier(0);
-    rhs_lo_to_hi_from<C + G, NC>(V, S, r);
+    rhs_lo_to_hi_from<C + G, NC>(V, S, hm);
   }
 }
 template <int NC, class T>
-__device__ __forceinline__ void rhs_lo_to_hi(T (&V)[NC], double (&S)[32], int r) {
-  rhs_lo_to_hi_from<0, NC>(V, S, r);
+__device__ __forceinline__ void rhs_lo_to_hi(T (&V)[NC], double (&S)[32], double hm) {
+  rhs_lo_to_hi_from<0, NC>(V, S, hm);
 }
 
 // Step k of the unit LDL^T factorisation of the rows kernel, with the rows of
@@ -1234,7 +1234,7 @@ __device__ __forceinline__ void rhs_lo_to_hi(T (&V)[NC], double (&S)[32], int r)
 // right-looking sweep.
 template <int k, int NC>
 __device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[NC], double& pr,
-                                         bool& fail, int r, int K) {
+                                         bool& fail, int r, int K, double lo, double hm) {
   double src;
   if constexpr (k < 16)
     src = from_lo_row(S[k]);
@@ -1244,19 +1244,22 @@ __device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[NC], doub
   fail |= (k < K) && !(pk > 0.0);
   const double ik = rcp_nr(pk);
   if (r == k) pr = pk;
-  const double lrk = S[k] * ik;
+  // multipliers of rows below k only: rows r <= k get 0, so their strict
+  // upper part stays zero and the diagonal entry is cleared (the pivot is
+  // kept in pr); later masks reduce to products with S
+  const double lrk = S[k] * ((r > k) ? ik : 0.0);
   if constexpr (k < 15) fac_trail<k + 1, 16, 0>(S, src, lrk);
   fac_trail<(k + 1 > 16 ? k + 1 : 16), 32, 16>(S, S[k], lrk);
-  S[k] = (r > k) ? lrk : S[k];
-  asm volatile("" : "+v"(S[k]));   // the select here, not deferred with both values live
+  S[k] = lrk;
+  asm volatile("" : "+v"(S[k]));
   if constexpr (k < 16) {
-    rhs_self<k>(bv, (r > k && r < 16) ? lrk : 0.0);
-    if constexpr (k == 15) rhs_lo_to_hi(bv, S, r);
+    rhs_self<k>(bv, lrk * lo);   // rows 16..31 take steps 0..15 after step 15
+    if constexpr (k == 15) rhs_lo_to_hi(bv, S, hm);
   } else {
-    rhs_self<k - 16>(bv, (r > k) ? lrk : 0.0);
+    rhs_self<k - 16>(bv, lrk);
   }
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (k + 1 < 32) fac_step<k + 1>(S, bv, pr, fail, r, K);
+  if constexpr (k + 1 < 32) fac_step<k + 1>(S, bv, pr, fail, r, K, lo, hm);
 }
 
 // W = L'^-1 R for the rows kernel's NC columns (unit lower L' in S), in
@@ -1266,28 +1269,28 @@ __device__ __forceinline__ void fac_step(double (&S)[32], double (&bv)[NC], doub
 // (S[K] passed through an empty asm there): computed ahead, all 31 would be
 // live at once.
 template <int K, int NC, class T>
-__device__ __forceinline__ void wsw_lo(T (&R)[NC], double (&S)[32], int r) {
+__device__ __forceinline__ void wsw_lo(T (&R)[NC], double (&S)[32], double lo) {
   if constexpr (K < 15) {
     asm volatile("" : "+v"(S[K]));
-    rhs_self<K>(R, (r > K && r < 16) ? (T)S[K] : (T)0);
+    rhs_self<K>(R, (T)(S[K] * lo));   // S[K] is zero on rows r <= K
     __builtin_amdgcn_sched_barrier(0);
-    wsw_lo<K + 1>(R, S, r);
+    wsw_lo<K + 1>(R, S, lo);
   }
 }
 template <int K, int NC, class T>
-__device__ __forceinline__ void wsw_hi(T (&R)[NC], double (&S)[32], int r) {
+__device__ __forceinline__ void wsw_hi(T (&R)[NC], double (&S)[32]) {
   if constexpr (K < 31) {
     asm volatile("" : "+v"(S[K]));
-    rhs_self<K - 16>(R, (r > K) ? (T)S[K] : (T)0);
+    rhs_self<K - 16>(R, (T)S[K]);   // zero on rows r <= K
     __builtin_amdgcn_sched_barrier(0);
-    wsw_hi<K + 1>(R, S, r);
+    wsw_hi<K + 1>(R, S);
   }
 }
 template <int NC, class T>
-__device__ __forceinline__ void w_sweep_dpp(T (&R)[NC], double (&S)[32], int r) {
-  wsw_lo<0>(R, S, r);
-  rhs_lo_to_hi(R, S, r);
-  wsw_hi<16>(R, S, r);
+__device__ __forceinline__ void w_sweep_dpp(T (&R)[NC], double (&S)[32], double lo, double hm) {
+  wsw_lo<0>(R, S, lo);
+  rhs_lo_to_hi(R, S, hm);
+  wsw_hi<16>(R, S);
 }
 
 template <int NU, bool MIX, int DM, bool PROF>
@@ -1451,8 +1454,10 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   // cross-lane values by DPP row broadcasts (fac_step)
   double pr = 1.0;
   bool fail = false;
+  // row masks: 1 on rows 0..15 (lo) or 16..31 (hm) of the half, else 0
+  const double lo = (opaque(r) < 16) ? 1.0 : 0.0, hm = 1.0 - lo;
   asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
-  fac_step<0>(S, bv, pr, fail, opaque(r), K);
+  fac_step<0>(S, bv, pr, fail, opaque(r), K, lo, hm);
   const double dinv = rsqrt_nr(pr);   // 1 / L_rr
 #pragma unroll
   for (int c = 0; c < NC; ++c) bv[c] *= dinv;   // rows of L^-1 [y Z]
@@ -1476,13 +1481,13 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     // for c >= 16 only rows 16..31 (the second 16-lane row) contribute
 #pragma unroll
     for (int c = 30; c >= 16; --c) {
-      const double s = row_sum((ra > c) ? S[c] * t : 0.0);
+      const double s = row_sum(S[c] * t);   // S[c] is zero on rows ra <= c
       if (ra == c) t -= s;
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int c = 15; c >= 0; --c) {
-      const double s = half_sum((ra > c) ? S[c] * t : 0.0);
+      const double s = half_sum(S[c] * t);
       if (ra == c) t -= s;
       __builtin_amdgcn_sched_barrier(0);
     }
@@ -1546,13 +1551,13 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
       for (int c = 0; c < NC; ++c) Rf[c] = (float)R[c];
       asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
-      w_sweep_dpp(Rf, S, opaque(r));
+      w_sweep_dpp(Rf, S, lo, hm);
       const float df = (float)dinv;
 #pragma unroll
       for (int c = 0; c < NC; ++c) R[c] = (double)(Rf[c] * df);
     } else {
       asm volatile("s_nop 4");   // EXEC restored by the branches above: DPP needs 5 wait states
-      w_sweep_dpp(R, S, opaque(r));
+      w_sweep_dpp(R, S, lo, hm);
 #pragma unroll
       for (int c = 0; c < NC; ++c) R[c] *= dinv;
     }
