@@ -1902,32 +1902,45 @@ __global__ __launch_bounds__(1024) void ll_sum_solve(const double* __restrict__ 
 // of the record head hit the same addresses).  Records are read by G lanes in
 // parallel instead of one thread walking 2 + q + 2P + Pq doubles.
 __global__ __launch_bounds__(256) void ll_pass2(const double* __restrict__ cache, int64_t nblk,
-                                                int P, int q, int need_grad, int G,
+                                                int P, int q, int need_grad, int G, int RPI,
                                                 const double* __restrict__ beta,
                                                 double* __restrict__ partial2) {
+  // RPI consecutive records per iteration are one contiguous span of the
+  // cache: copied to LDS by consecutive threads (coalesced), then lane group
+  // g (G lanes) evaluates record g from LDS.  beta sits in LDS.
+  extern __shared__ double srec[];   // RPI * NC records, then beta
   __shared__ double red[256];
   const int NC = need_grad ? 2 + q + 2 * P + P * q : 2 + q;
   const int NV = need_grad ? 1 + P : 1;
-  const int t = threadIdx.x & (G - 1);
-  const int64_t rstride = (int64_t)gridDim.x * (blockDim.x / G);
+  double* sb = srec + RPI * NC;
+  for (int c = threadIdx.x; c < q; c += 256) sb[c] = beta[c];
+  const int g = threadIdx.x / G, t = threadIdx.x & (G - 1);
   double acc = 0.0;
-  for (int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; b < nblk; b += rstride) {
-    const double* rec = cache + b * NC;
-    double zK = rec[0];
-    for (int c = 0; c < q; ++c) zK -= rec[1 + c] * beta[c];
-    if (t == 0) {
-      acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
-    } else if (t < NV) {
-      const int p = t - 1;
-      const double* wK = rec + 2 + q;
-      const double* sv = wK + P;
-      const double* Tm = sv + P + p * q;
-      double wz = sv[p];
-      for (int c = 0; c < q; ++c) wz -= Tm[c] * beta[c];
-      acc += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
+  for (int64_t b0 = (int64_t)blockIdx.x * RPI; b0 < nblk; b0 += (int64_t)gridDim.x * RPI) {
+    const int nr = (int)(nblk - b0 < RPI ? nblk - b0 : RPI);
+    const double* src = cache + b0 * NC;
+    __syncthreads();   // the previous records are consumed (and beta is in place)
+    for (int i = threadIdx.x; i < nr * NC; i += 256) srec[i] = src[i];
+    __syncthreads();
+    if (g < nr) {
+      const double* rec = srec + g * NC;
+      double zK = rec[0];
+      for (int c = 0; c < q; ++c) zK -= rec[1 + c] * sb[c];
+      if (t == 0) {
+        acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+      } else if (t < NV) {
+        const int p = t - 1;
+        const double* wK = rec + 2 + q;
+        const double* sv = wK + P;
+        const double* Tm = sv + P + p * q;
+        double wz = sv[p];
+        for (int c = 0; c < q; ++c) wz -= Tm[c] * sb[c];
+        acc += -0.5 * (wK[p] * (1.0 + zK * zK) - 2.0 * zK * wz);
+      }
     }
   }
   // sum over the workgroup's records, per lane-in-group t
+  __syncthreads();
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s >= G; s >>= 1) {
@@ -2750,7 +2763,12 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   if (need_grad)
     while (G2L < 1 + P) G2L <<= 1;
   SVH_REQUIRE(G2L <= 256, "pass 2: too many parameters");
-  const int G2 = (int)std::min<int64_t>((vs.nblk * G2L + 255) / 256, 2048);
+  // pass 2: RPI records per workgroup iteration (256 / G2L lane groups, at
+  // most 4096 doubles of records in LDS)
+  const int NCr = need_grad ? 2 + q + 2 * P + P * q : 2 + q;
+  const int RPI = std::max(1, std::min(256 / G2L, 4096 / NCr));
+  const int G2 = (int)std::min<int64_t>((vs.nblk + RPI - 1) / RPI, 2048);
+  const size_t p2lds = ((size_t)RPI * NCr + 33) * sizeof(double);
   const size_t lds = (size_t)(KP * d + KP * (KP + 1) + NR * KP + KP + NACC +
                              (est_nu ? KP * (KP + 1) : 0)) * sizeof(double) +
                      (size_t)KP * sizeof(int);
@@ -2874,7 +2892,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
       hipLaunchKernelGGL(ll_solve_beta, dim3(1), dim3(256), 0, s, tot, P, q, need_grad, dbeta,
                          dout);
     }
-    hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), 0, s, a.cache, vs.nblk, P, q, need_grad, G2L,
+    hipLaunchKernelGGL(ll_pass2, dim3(G2), dim3(256), p2lds, s, a.cache, vs.nblk, P, q, need_grad, G2L, RPI,
                        dbeta, partial2);
     hipLaunchKernelGGL(ll_reduce2, dim3(NV + 1), dim3(256), 0, s, partial2, G2, NV, fail, nchunk,
                        P, q, dout);
