@@ -2706,9 +2706,13 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
 
   const int nchunk = need_grad ? (est_nu ? vs.nchunk_grad_nu : vs.nchunk_grad) : vs.nchunk_val;
   // jitter levels and failure flags of the chunks, cleared by one launch
+  // unless the previous evaluation left this buffer zero (no escalation)
   int* level = ctx->arena.get_t<int>("v.levfail", 2 * (int64_t)nchunk);
   int* fail = level + nchunk;
-  int_fill(ctx, level, 0, 2 * (int64_t)nchunk);
+  if (ctx->arena.fix_base || ctx->levfail_zero != (const void*)level ||
+      ctx->levfail_zero_n < 2 * (int64_t)nchunk)
+    int_fill(ctx, level, 0, 2 * (int64_t)nchunk);
+  ctx->levfail_zero = nullptr;
 
   const int KP = vs.kp;
   const int NR = need_grad ? P + 1 + q : 1 + q;
@@ -2901,7 +2905,13 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
     SVH_HIP(hipStreamSynchronize(s));
     ctx->timers.collect();
     int nfail = (int)hout[NOUT - 1];
-    if (nfail == 0) break;
+    if (nfail == 0) {
+      if (rounds == 0 && !ctx->arena.fix_base) {
+        ctx->levfail_zero = level;
+        ctx->levfail_zero_n = 2 * (int64_t)nchunk;
+      }
+      break;
+    }
     if (rounds == 9) throw SvhError{SVH_ERR_CHOLESKY, "block Cholesky failed even with jitter"};
     hipLaunchKernelGGL(ll_escalate, dim3((nchunk + 255) / 256), dim3(256), 0, s, level, fail,
                        nchunk);

@@ -200,6 +200,10 @@ struct svh_ctx {
   int64_t rb_n = 0;   // sizes of the estimation data resident for rebuilds
   int rb_d = 0, rb_q = 0, rb_zmode = 0;
   bool batch_data = false;   // unordered batch data uploaded for the current batch
+  // the evaluation's chunk levels and failure flags ("v.levfail"): the buffer
+  // and length last left all zero (an evaluation without jitter escalation)
+  const void* levfail_zero = nullptr;
+  int64_t levfail_zero_n = 0;
   MlpState* mlp = nullptr;
   Autotune tune;
   void* scratch = nullptr;   // this device's fixed working array (api.hip), or nullptr

@@ -221,6 +221,36 @@ def test_loglik_jitter_ladder_matches_oracle(gpu):
     assert _rel(F, F0) < 1e-4
 
 
+def test_loglik_levels_reset_after_jitter(gpu):
+    """Chunk levels raised by an escalating evaluation do not leak into the
+    next one on the same context; evaluations that follow a clean one (the
+    clearing launch skipped) give the same results bit for bit."""
+    from pyscaledvecchia_amd.likelihood import vecchia_profile_loglik
+    rng = np.random.default_rng(22)
+    n, d, m = 300, 2, 12
+    X = rng.random((n, d))
+    y = rng.standard_normal(n)
+    th_sing = np.array([0.0, math.log(0.3), math.log(0.3), -np.inf])
+    th_pos = np.array([0.0, math.log(0.3), math.log(0.3), math.log(1e-3)])
+    order, groups = _structure(X, th_sing, m)
+    rows, nb = groups[-1]
+    nb = nb.copy()
+    nb[100, 1] = nb[100, 0]
+    groups = groups[:-1] + [(rows, nb)]
+    data = (X[order], y[order], np.ones((n, 1)), groups, 2.5)
+    r1 = vecchia_profile_loglik(th_pos, *data)
+    assert gpu.last_jitter_power == 0
+    vecchia_profile_loglik(th_sing, *data)
+    assert gpu.last_jitter_power == -12
+    r3 = vecchia_profile_loglik(th_pos, *data)
+    assert gpu.last_jitter_power == 0
+    r4 = vecchia_profile_loglik(th_pos, *data)
+    for a, b in ((r1, r3), (r1, r4)):
+        assert a[0] == b[0]
+        for u, v in zip(a[1:], b[1:]):
+            np.testing.assert_array_equal(u, v)
+
+
 def test_loglik_c2_shape_against_oracle(gpu):
     """Full C2 dimensions (d=8, linear trend, m=30) on a 6000-row slice."""
     from pyscaledvecchia_amd import VecchiaStructure
