@@ -986,9 +986,10 @@ __device__ __forceinline__ double half_sum(double v) {
 }
 
 // LDS doubles per wave of the rows kernel: per half [ab 32 | xs 32*dm], then
-// the packed lower triangles of g(r) of both halves (2 x 496; the product
-// operands Mt, 24 x 33, reuse that space once g is consumed), then misc (16).
-constexpr int kGTri = 496;   // 32 * 31 / 2 strictly-lower entries
+// the packed lower triangles (diagonal included) of Sigma, later g(r), of
+// both halves (2 x 528; the product operands Mt, 24 x 33, reuse that space
+// once g is consumed), then misc (16).
+constexpr int kGTri = 528;   // 32 * 33 / 2 lower entries
 // Coordinate registers of the rows kernel for d inputs and q trend columns:
 // the kernel carries DM + 2 right-hand-side columns, which must hold both
 // [y Z] (1 + q) and the d + 2 derivative columns.
@@ -1000,11 +1001,12 @@ __host__ __device__ inline int rows_half_doubles(int dm) { return 32 + 32 * dm; 
 __host__ __device__ inline int rows_lds_doubles(int dm) {
   return 2 * rows_half_doubles(dm) + 2 * kGTri + 16;
 }
-// Entry (i, j), i > j, of a strictly-lower 32 x 32 triangle packed by
-// columns: column j holds rows j+1..31 contiguously, so g_base(j) + i.
-__host__ __device__ constexpr int g_base(int j) { return 30 * j - (j * (j - 1)) / 2 - 1; }
+// Entry (i, j), i >= j, of a lower 32 x 32 triangle packed by columns:
+// column j holds rows j..31 contiguously, so g_base(j) + i.  For i < j the
+// same expression stays inside the array (0 .. 526): another entry.
+__host__ __device__ constexpr int g_base(int j) { return 31 * j - (j * (j - 1)) / 2; }
 // The same for run-time indices 0 <= j <= 31 (24-bit multiply, full rate).
-__device__ __forceinline__ int g_base_rt(int j) { return (int)(__umul24((unsigned)j, (unsigned)(61 - j)) >> 1) - 1; }
+__device__ __forceinline__ int g_base_rt(int j) { return (int)(__umul24((unsigned)j, (unsigned)(63 - j)) >> 1); }
 // Coordinate l of point p of a block in its 32 x dm slot: pairs of
 // coordinates interleaved by point, so a lane's two coordinates are one
 // 16-byte read and 32 lanes reading 32 points touch distinct banks.
@@ -1422,19 +1424,22 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
       bv[1 + c] = z;
     }
   }
+  // the diagonal of this half's block (every slot of the triangle is now
+  // written: the 496 pairs and the 32 diagonal entries)
+  {
+    const int rr = opaque(r);
+    gh[g_base_rt(rr) + rr] = act ? var * (1.0 + nug) + jit : 1.0;
+  }
   wave_sync();
-  // ---- Sigma row r of this half's block, lower part (upper part zero)
+  // ---- Sigma row r of this half's block: the lower part and the diagonal
+  // from the triangle, the upper part other (finite) entries of it -- the
+  // factorisation clears the upper part at each step without reading it
+  // (its multipliers are 0 there)
   double S[32];
   {
     const int rr = opaque(r);
-    const double dg = act ? var * (1.0 + nug) + jit : 1.0;
 #pragma unroll
-    for (int c = 0; c < 31; ++c) {
-      const int gi = g_base(c) + rr;
-      const double v = gh[gi > 0 ? gi : 0];
-      S[c] = (c < rr) ? v : (c == rr ? dg : 0.0);
-    }
-    S[31] = (rr == 31) ? dg : 0.0;
+    for (int c = 0; c < 32; ++c) S[c] = gh[g_base(c) + rr];
   }
   wave_sync();   // every row is read before g takes the triangle
   {
@@ -1496,8 +1501,8 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     wave_sync();
     SVH_PHASE(3)
     // ---- dS_p a, row r, with the g(r) stored by the Sigma phase: entry
-    // (max(r, c), min(r, c)) of the packed triangle (c = r reads a finite
-    // neighbour entry and multiplies a zero difference)
+    // (max(r, c), min(r, c)) of the packed triangle (c = r reads the finite
+    // diagonal slot and multiplies a zero difference)
     const int rd = opaque(r);
     const int gb_r = g_base(rd);
     double x[DM];
@@ -1513,7 +1518,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll 2
       for (int c = 0; c < 32; ++c) {
         const float ac = (float)ab[c];
-        const int gi = (c < rd) ? g_base(c) + rd : (c > rd ? gb_r + c : 0);
+        const int gi = (c < rd) ? g_base(c) + rd : gb_r + c;
         const float g = (float)gh[gi];
         float dl[DM];
 #pragma unroll
@@ -1529,7 +1534,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
       for (int c = 0; c < 32; ++c) {
         const double ac = ab[c];
-        const int gi = (c < rd) ? g_base(c) + rd : (c > rd ? gb_r + c : 0);
+        const int gi = (c < rd) ? g_base(c) + rd : gb_r + c;
         const double g = gh[gi];
         double dl[DM];
 #pragma unroll
