@@ -990,6 +990,8 @@ __device__ __forceinline__ double half_sum(double v) {
 // both halves (2 x 528; the product operands Mt, 24 x 33, reuse that space
 // once g is consumed), then misc (16).
 constexpr int kGTri = 528;   // 32 * 33 / 2 lower entries
+// first coordinate of padding rows (scaled inputs are at most 1e8 in size)
+constexpr double kPadFar = 1e17;
 // Coordinate registers of the rows kernel for d inputs and q trend columns:
 // the kernel carries DM + 2 right-hand-side columns, which must hold both
 // [y Z] (1 + q) and the d + 2 derivative columns.
@@ -1323,13 +1325,18 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   const bool act = bvalid && r < K;
   int pt = 0;
   if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
+  // Padding rows (r >= K) sit far apart from everything: first coordinate
+  // kPadFar * (r + 1), the others 0.  Every pair with a padding point is then
+  // at least kPadFar apart, so its covariance and g(r) underflow to exactly 0
+  // (e^-c with c clamped, times a finite polynomial; squares stay finite in
+  // float32 too) and need no mask.
 #pragma unroll
   for (int l = 0; l < DM; ++l) {
     const double raw = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
-    xs[xs_at(l, r)] = (l < d) ? raw * a.inv_lam[l] : 0.0;
+    double v = (l < d) ? raw * a.inv_lam[l] : 0.0;
+    if (l == 0) v = act ? v : kPadFar * (double)(r + 1);
+    xs[xs_at(l, r)] = v;
   }
-  const int Kb0 = (b0 < a.nblk) ? a.kcnt[b0] + 1 : 1;           // K of both blocks
-  const int Kb1 = (b0 + 1 < a.nblk) ? a.kcnt[b0 + 1] + 1 : 1;
   const double yv = act ? a.y[pt] : 0.0;
   const double var = a.var, nug = a.nug;
   double jit = 0.0;
@@ -1351,7 +1358,6 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
     const int rr = opaque(lane_o & 31), e = lane_o >> 5;
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh) {
-      const int Kb = hh == 0 ? Kb0 : Kb1;
       const double* xh = lds + hh * rows_half_doubles(DM) + 32;   // block hh's coordinates
       double xi[DM];
 #pragma unroll
@@ -1389,10 +1395,6 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
             gs = g;
           }
         }
-        // pairs with a padding point (zero coordinates, finite values) are zero
-        const bool real = i < Kb;
-        v = real ? v : 0.0;
-        gs = real ? gs : 0.0;
         // t = 16 (u = 7, upper half of the wave) is stored by rows 0..15
         // only; u < 7 gives t <= 15 and needs no test (no branch)
         if (u < 7 || t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = v;
