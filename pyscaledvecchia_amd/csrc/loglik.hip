@@ -985,7 +985,8 @@ __device__ __forceinline__ double half_sum(double v) {
   return a0 + a1;
 }
 
-// LDS doubles per wave of the rows kernel: per half [ab 32 | xs 32*dm], then
+// LDS doubles per wave of the rows kernel: per half [ab 32 | xs 32*dm |
+// squared centred coordinates 32*dm], then
 // the packed lower triangles (diagonal included) of Sigma, later g(r), of
 // both halves (2 x 528; the product operands Mt, 24 x 33, reuse that space
 // once g is consumed), then misc (16).
@@ -999,7 +1000,7 @@ __host__ __device__ inline int rows_dm(int d, int q) {
   const int k = d > q - 1 ? d : q - 1;
   return k <= 4 ? 4 : (k <= 8 ? 8 : 10);
 }
-__host__ __device__ inline int rows_half_doubles(int dm) { return 32 + 32 * dm; }
+__host__ __device__ inline int rows_half_doubles(int dm) { return 32 + 64 * dm; }
 __host__ __device__ inline int rows_lds_doubles(int dm) {
   return 2 * rows_half_doubles(dm) + 2 * kGTri + 16;
 }
@@ -1455,6 +1456,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
         const int i = rr > c ? rr : c, j = rr > c ? c : rr;
         if (u < 7 || t < 16 || rr < 16) gt[hh * kGTri + g_base_rt(j) + i] = gk[8 * hh + u];
       }
+    gh[g_base_rt(rr) + rr] = 0.0;   // g(r, r)
   }
   SVH_PHASE(1)
   // ---- factorisation (unit lower L', pivots) with L'^-1 [y Z] on the fly,
@@ -1533,21 +1535,54 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
 #pragma unroll
       for (int l = 0; l < DM; ++l) R[1 + l] = act ? -var * (double)Rf[l] : 0.0;
     } else {
+      // sum_c w_c (x_rl - x_cl)^2, w_c = g(r, c) a_c, expanded about the
+      // block's own point (x~ = x - x_{K-1}, small within a block):
+      // x~_rl^2 W0 - 2 x~_rl W1_l + W2_l, W0 = sum w, W1 = sum w x~_c,
+      // W2 = sum w x~_c^2, the centred coordinates and their squares in LDS
+      // (every lane reads the same c: broadcasts).  g(r, r) is 0.
+      double* xs2 = xs + 32 * DM;
+      {
+        double xt[DM];
+#pragma unroll
+        for (int l = 0; l < DM; ++l) xt[l] = x[l] - xs[xs_at(l, K - 1)];
+        wave_sync();   // the block's own point is read before the slots change
+#pragma unroll
+        for (int l = 0; l < DM; ++l) {
+          xs[xs_at(l, rd)] = xt[l];
+          xs2[xs_at(l, rd)] = xt[l] * xt[l];
+        }
+      }
+      wave_sync();
+      double W0 = 0.0, W1[DM], W2[DM];
+#pragma unroll
+      for (int l = 0; l < DM; ++l) W1[l] = W2[l] = 0.0;
+      // coordinates l, l + 1 of point c are adjacent (16-byte aligned pairs):
+      // one 128-bit read each, constant offsets from one base
+      typedef double dpair __attribute__((ext_vector_type(2)));
+      const dpair* xp = reinterpret_cast<const dpair*>(xs);
+      const dpair* xp2 = reinterpret_cast<const dpair*>(xs2);
 #pragma unroll
       for (int c = 0; c < 32; ++c) {
         const double ac = ab[c];
         const int gi = (c < rd) ? g_base(c) + rd : gb_r + c;
-        const double g = gh[gi];
-        double dl[DM];
+        const double w = gh[gi] * ac;
+        W0 += w;
 #pragma unroll
-        for (int l = 0; l < DM; ++l) dl[l] = x[l] - xs[xs_at(l, c)];
-        const double w = g * ac;
-#pragma unroll
-        for (int l = 0; l < DM; ++l) R[1 + l] = fma(w, dl[l] * dl[l], R[1 + l]);
+        for (int l = 0; l < DM; l += 2) {
+          const dpair v = xp[xs_at(l, c) / 2], v2 = xp2[xs_at(l, c) / 2];
+          W1[l] = fma(w, v.x, W1[l]);
+          W1[l + 1] = fma(w, v.y, W1[l + 1]);
+          W2[l] = fma(w, v2.x, W2[l]);
+          W2[l + 1] = fma(w, v2.y, W2[l + 1]);
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
-      for (int l = 0; l < DM; ++l) R[1 + l] = act ? R[1 + l] * -var : 0.0;
+      for (int l = 0; l < DM; ++l) {
+        const double xr = xs[xs_at(l, rd)];
+        const double v = fma(xr, fma(xr, W0, -2.0 * W1[l]), W2[l]);
+        R[1 + l] = act ? v * -var : 0.0;
+      }
     }
     R[0] = act ? ((r == K - 1) ? pK * dK : 0.0) - jit * av : 0.0;
     R[NC - 1] = act ? var * nug * av : 0.0;
