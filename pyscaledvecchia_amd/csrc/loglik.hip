@@ -29,7 +29,9 @@
 // log-determinant, a = L^-T e_K, L^-1 [y Z] and the GLS solve are FP64 in
 // every mode.
 //
-// Cached record per block (need_grad):  u, v[q], log L_KK, wK[P], s[P], T[P*q]
+// Cached record per block (need_grad):  u, v[q], L_KK^2, wK[P], s[P], T[P*q]
+// (the logarithm of L_KK^2 is taken in pass 2, where it is batched over
+// records instead of costing a whole wave per block in pass 1)
 // where u = (L^-1 y)_K, v = (L^-1 Z)_K, W_p = L^-1 dS_p a, s_p = W_p.(L^-1 y),
 // T_pc = W_p.(L^-1 Z_c).  Then z = L^-1 y - (L^-1 Z) beta gives
 // z_K = u - v.beta and W_p.z = s_p - T_p.beta exactly (linearity of L^-1).
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(kWave) void ll_pass1(LLArgs a) {
       else if (e <= q)
         v = R[(cy + e) * KP + km];
       else
-        v = log(S[km * LD + km]);
+        v = S[km * LD + km] * S[km * LD + km];   // L_KK^2
       rec[e] = v;
     }
     // GLS normal equations
@@ -677,7 +679,7 @@ __device__ __forceinline__ void split_block(const LLArgs& a, int64_t b, double* 
   if (r == K - 1 && h == 0) {
     Misc[0] = dinv;
     Misc[1] = piv;
-    Misc[2] = 0.5 * log(piv);
+    Misc[2] = piv;   // L_KK^2
   }
   __syncthreads();
   SVH_PHASE(2)
@@ -1474,7 +1476,7 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   if (r == K - 1) {
     misc[8 * h + 1] = pr;
     misc[8 * h + 2] = dinv;
-    misc[8 * h + 3] = 0.5 * log(pr);   // log L_KK of the block record
+    misc[8 * h + 3] = pr;   // L_KK^2 of the block record
   }
   if (r == 31) misc[8 * h] = fail ? 1.0 : 0.0;   // the second 16-lane row saw every pivot
   wave_sync();
@@ -1964,12 +1966,15 @@ __global__ __launch_bounds__(256) void ll_pass2(const double* __restrict__ cache
     __syncthreads();   // the previous records are consumed (and beta is in place)
     for (int i = threadIdx.x; i < nr * NC; i += 256) srec[i] = src[i];
     __syncthreads();
+    // log L_KK^2 of the staged records, one record per lane of the first wave
+    if (threadIdx.x < nr) srec[threadIdx.x * NC + 1 + q] = log(srec[threadIdx.x * NC + 1 + q]);
+    __syncthreads();
     if (g < nr) {
       const double* rec = srec + g * NC;
       double zK = rec[0];
       for (int c = 0; c < q; ++c) zK -= rec[1 + c] * sb[c];
       if (t == 0) {
-        acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+        acc += -0.5 * (rec[1 + q] + zK * zK);
       } else if (t < NV) {
         const int p = t - 1;
         const double* wK = rec + 2 + q;
@@ -2232,7 +2237,7 @@ __global__ __launch_bounds__(256) void ll_pass2_batch(const DSDesc* __restrict__
     double zK = rec[0];
     for (int c = 0; c < q; ++c) zK -= rec[1 + c] * bet[c];
     if (t == 0) {
-      acc += -0.5 * (2.0 * rec[1 + q] + zK * zK);
+      acc += -0.5 * (log(rec[1 + q]) + zK * zK);
     } else if (t <= P) {
       const int p = t - 1;
       const double* wK = rec + 2 + q;
