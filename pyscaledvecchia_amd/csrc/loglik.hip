@@ -1858,12 +1858,13 @@ __global__ __launch_bounds__(256) void ll_solve_beta(const double* __restrict__ 
   ll_solve_beta_body(tot, P, q, need_grad, beta, out, P, P);
 }
 
-// beta = solve(XtX + 1e-12 I, Zty), q <= 32, by one wave (lane i holds row i of the
-// augmented system in LDS): the pivot (first largest |A[i][k]|, i >= k) by a
-// wave arg-max, rows exchanged column-parallel, each row eliminated by its
-// lane with A[i][j] -= (A[i][k] / A[k][k]) * A[k][j], back substitution by
-// columns (x_c formed, then every row above subtracts A[r][c] x_c).  The
-// caller has written the Fisher block of `out`.
+// beta = solve(XtX + 1e-12 I, Zty), q <= 32, by one wave (lane i holds row i
+// of the augmented system in LDS): the pivot (first largest |A[i][k]|,
+// i >= k) found by every lane from broadcast reads, rows exchanged
+// column-parallel, each row eliminated by its lane with
+// A[i][j] -= (A[i][k] / A[k][k]) * A[k][j], back substitution by columns
+// (x_c formed, then every row above subtracts A[r][c] x_c).  The caller has
+// written the Fisher block of `out`.
 __device__ void ll_solve_beta_wave(const double* __restrict__ tot, int P, int q, int need_grad,
                                    double* __restrict__ beta, double* __restrict__ out) {
   __shared__ double A[32][34];
@@ -1875,14 +1876,15 @@ __device__ void ll_solve_beta_wave(const double* __restrict__ tot, int P, int q,
   }
   wave_sync();
   for (int k = 0; k < q; ++k) {
-    double v = (lane >= k && lane < q) ? fabs(A[lane][k]) : -1.0;
-    int p = lane;
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ov = __shfl_xor(v, off, kWave);
-      const int op = __shfl_xor(p, off, kWave);
-      if (ov > v || (ov == v && op < p)) {
-        v = ov;
-        p = op;
+    // pivot: every lane scans column k (the same LDS addresses: broadcast
+    // reads) for the first largest |A[i][k]|, i >= k
+    int p = k;
+    double best = fabs(A[k][k]);
+    for (int i = k + 1; i < q; ++i) {
+      const double v = fabs(A[i][k]);
+      if (v > best) {
+        best = v;
+        p = i;
       }
     }
     if (p != k && lane <= q) {
@@ -1893,6 +1895,7 @@ __device__ void ll_solve_beta_wave(const double* __restrict__ tot, int P, int q,
     wave_sync();
     if (lane > k && lane < q) {
       const double f = A[lane][k] / A[k][k];
+#pragma unroll 4
       for (int j = k + 1; j <= q; ++j) A[lane][j] -= f * A[k][j];
     }
     wave_sync();
