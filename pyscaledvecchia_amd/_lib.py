@@ -318,23 +318,46 @@ class Context:
         self._q = q
         return order
 
+    def _eval_buffers(self, P):
+        """One host array per (P, q) for an evaluation's input and outputs,
+        [theta | ll | grad | Fisher | beta], with its ctypes pointers made once
+        (building pointer objects costs more than the copies per call)."""
+        q = self._q
+        key = (P, q)
+        cache = getattr(self, "_evbuf", None)
+        if cache is None:
+            cache = self._evbuf = {}
+        ent = cache.get(key)
+        if ent is None:
+            arr = np.zeros(P + 1 + P + P * P + max(q, 1))
+            base = arr.ctypes.data
+            offs = (0, P, P + 1, 2 * P + 1, 2 * P + 1 + P * P)
+            ptrs = tuple(ctypes.cast(base + 8 * o, _dp) for o in offs)
+            jp = ctypes.c_int32(0)
+            ent = cache[key] = (arr, offs, ptrs, jp, ctypes.byref(jp))
+        return ent
+
     def vecchia_eval(self, theta, nu, need_grad=True, var_penalty=0.0, log_var_target=0.0,
                      precision="fp64"):
         theta = _f64(theta)
+        if theta.ndim != 1:
+            raise ValueError("theta must be 1-D")
         P = theta.shape[0]
-        ll = np.zeros(1)
-        grad = np.zeros(P)
-        fish = np.zeros((P, P))
-        beta = np.zeros(self._q)
-        jp = ctypes.c_int32(0)
+        q = self._q
+        arr, offs, ptrs, jp, jref = self._eval_buffers(P)
+        arr[:P] = theta
         self._check(self._L.svh_vecchia_eval(
-            self._h, _ptr(theta), P, float(nu), PRECISION[precision], int(bool(need_grad)),
-            float(var_penalty), float(log_var_target), _ptr(ll), _ptr(grad), _ptr(fish),
-            _ptr(beta) if self._q else None, ctypes.byref(jp)))
+            self._h, ptrs[0], P, float(nu), PRECISION[precision], int(bool(need_grad)),
+            float(var_penalty), float(log_var_target), ptrs[1], ptrs[2], ptrs[3],
+            ptrs[4] if q else None, jref))
         self.last_jitter_power = jp.value
+        ll = float(arr[offs[1]])
+        beta = arr[offs[4]:offs[4] + q].copy()
         if not need_grad:
-            return float(ll[0]), None, None, beta
-        return float(ll[0]), grad, fish, beta
+            return ll, None, None, beta
+        grad = arr[offs[2]:offs[2] + P].copy()
+        fish = arr[offs[3]:offs[3] + P * P].reshape(P, P).copy()
+        return ll, grad, fish, beta
 
     # ---------------------------------------------------------------- (c)
     def predict(self, Xtr, resid, Xq, m, theta, nu, noise_free=True, precision="fp64",
