@@ -128,13 +128,16 @@ __global__ void to_soa_kernel(const double* __restrict__ X, int64_t n, int d,
 // coordinates zero: each extra term adds 0*0 = +0 to a non-negative sum, so
 // the squared distances are bit-identical to the d-term sums.
 constexpr int kSlice = 256;
-// published candidates per slice (TOP): 4, or 16 when there are at most
-// kDeepSlices slices (n <= 24,576): small inputs have few slices, and a
-// deeper list per slice gives fewer, fuller rounds (measured, d = 8: 16 is
-// faster up to about 24k points, 4 from about 30k, where the larger pool
-// makes each selection step dearer)
+// published candidates per slice (TOP): 16 when there are at most
+// kDeepSlices slices (n <= 8,192), 8 up to kMidSlices (n <= 28,672), else 4:
+// small inputs have few slices, and a deeper list per slice gives fewer,
+// fuller rounds, while more survivors make each round's set-up dearer
+// (measured at d = 4 and 8, same orders at every depth: n = 3000 16 1.6 ms /
+// 8 2.3 / 4 3.5; n = 12,000 4.9 / 3.9 / 5.9; n = 20,000 7.9 / 6.1 / 6.8;
+// n = 50,000 25.6 / 16.6 / 12.2)
 constexpr int kSliceTopMax = 16;
-constexpr int kDeepSlices = 96;
+constexpr int kDeepSlices = 32;
+constexpr int kMidSlices = 112;
 constexpr int kPool = 256;            // pool positions = select threads
 constexpr int kAccCap = kPool;        // max accepted points per round
 constexpr int kSortCap = 4096;        // survivors kept for the pool sort
@@ -1587,12 +1590,12 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   SVH_REQUIRE(dmax <= kMMD, "maximin: d above 12 not supported");
   // coordinate registers per point: d rounded up to 4, 8, 10 or 12 (zero padded)
   const int dm = dmax <= 4 ? 4 : (dmax <= 8 ? 8 : (dmax <= 10 ? 10 : 12));
-  // published list depth: deep lists for few slices (SVH_MM_TOP = 4 / 16
-  // overrides, for tests)
-  int top = maxslice <= kDeepSlices ? 16 : 4;
+  // published list depth: deeper lists for fewer slices (SVH_MM_TOP = 4, 8
+  // or 16 overrides, for tests)
+  int top = maxslice <= kDeepSlices ? 16 : (maxslice <= kMidSlices ? 8 : 4);
   if (const char* e = std::getenv("SVH_MM_TOP")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 16) top = v;
+    if (v == 4 || v == 8 || v == 16) top = v;
   }
   int sortcap = 1;
   while (sortcap < maxslice * top && sortcap < kSortCap) sortcap <<= 1;
@@ -1625,6 +1628,8 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
                       dim3(maxslice, nd), nd};
       if (top == 16)
         mm_round_dm<16>(ctx, dm, R);
+      else if (top == 8)
+        mm_round_dm<8>(ctx, dm, R);
       else
         mm_round_dm<4>(ctx, dm, R);
     }

@@ -83,11 +83,11 @@ def test_maximin_survivor_overflow_stays_exact(gpu, monkeypatch):
     assert gpu.timing_read("maximin_fallbacks")[1] > 0
 
 
-@pytest.mark.parametrize("top", ["4", "16"])
+@pytest.mark.parametrize("top", ["4", "8", "16"])
 def test_maximin_both_list_depths_exact(gpu, monkeypatch, top):
-    """Slices publish 4 or 16 candidates (16 by default up to 96 slices);
-    both depths give the serial order, on random points and on a grid with
-    exact ties."""
+    """Slices publish 4, 8 or 16 candidates (by default 16 up to 32 slices,
+    8 up to 112); every depth gives the serial order, on random points and on
+    a grid with exact ties."""
     from oracle import c_oracle as C
     monkeypatch.setenv("SVH_MM_TOP", top)
     X = np.random.default_rng(31).random((20000, 5)) / 0.3
