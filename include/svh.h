@@ -115,7 +115,10 @@ int svh_vecchia_setup(svh_ctx* ctx, const double* Xo, const double* yo, const do
  * data Xe (n x d), ye (n), Ze (n x q, NULL when q == 0) -- uploaded when
  * upload_data is nonzero, otherwise the copy of the previous call with the
  * same n, d, q is reused -- scaled by lam (d positive ranges, X / lam as the
- * host computes it), the maximin order from `first` (svh_maximin), the
+ * host computes it), the maximin order from `first` (svh_maximin; first = -1:
+ * the row closest to the scaled centroid, ordering.py:72, found on the device
+ * with the host's NumPy arithmetic -- column means summed in row order,
+ * pairwise row sums -- so the same index), the
  * ordered m nearest neighbours of the reordered scaled rows (svh_ordered_knn,
  * start 0), and the structure svh_vecchia_setup would receive for the groups
  * of ordering.py:119-127: blocks in row order, block i conditioning row i on

@@ -2658,7 +2658,7 @@ void vecchia_rebuild_device(svh_ctx* ctx, const double* Xe, const double* ye, co
   SVH_REQUIRE(n > 0 && d > 0 && d <= 32 && q >= 0 && q <= 32, "rebuild: bad n/d/q");
   SVH_REQUIRE(n < (1ll << 29), "rebuild: n too large");
   SVH_REQUIRE(m >= 0 && m < n && m + 1 <= kMaxKP, "rebuild: need 0 <= m < n, m <= 126");
-  SVH_REQUIRE(first >= 0 && first < n, "rebuild: first out of range");
+  SVH_REQUIRE(first >= -1 && first < n, "rebuild: first out of range");   // -1: on the device
   hipStream_t s = ctx->stream;
   // unordered estimation data stay resident between rebuilds of one fit
   double* dXe = ctx->arena.get_t<double>("rb.X", (size_t)n * d);

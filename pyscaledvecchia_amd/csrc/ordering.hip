@@ -1560,8 +1560,111 @@ void mm_round_dm(svh_ctx* ctx, int dm, const MMRound& R) {
     mm_round<12, TOP>(ctx, R);
 }
 
+// The first point of the maximin order on the device: the row closest to the
+// centroid, argmin(((X - X.mean(0)) ** 2).sum(1)) (ordering.py:72), with
+// NumPy's arithmetic for a C-ordered float64 array: the column sums of
+// mean(0) run over the rows in order, divided by n; (x - mu)^2 is a
+// subtraction and a product (no contraction); the row sum is NumPy's pairwise
+// sum over d <= 12 terms (fewer than 8: in order from 0; else the tree
+// ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7)), then t8.. in order);
+// the first index of the smallest value wins.
+// One workgroup per coordinate: the column streams through two LDS buffers
+// (all 256 threads load the next chunk while thread 0 adds the current one),
+// so the sequential add chain waits on LDS, not on global-memory latency.
+constexpr int kMeanChunk = 2048;   // doubles per LDS buffer (16 KB)
+
+__global__ __launch_bounds__(256) void mm_first_mean_kernel(const double* __restrict__ XT, int n,
+                                                            double* __restrict__ mu) {
+  __shared__ double buf[2][kMeanChunk];
+  const double* c = XT + (int64_t)blockIdx.x * n;
+  const int nch = (n + kMeanChunk - 1) / kMeanChunk;
+  double r[kMeanChunk / 256];
+  auto load = [&](int ch) {
+#pragma unroll
+    for (int j = 0; j < kMeanChunk / 256; ++j) {
+      const int i = ch * kMeanChunk + j * 256 + (int)threadIdx.x;
+      r[j] = i < n ? c[i] : 0.0;
+    }
+  };
+  auto stage = [&](int ch) {
+#pragma unroll
+    for (int j = 0; j < kMeanChunk / 256; ++j) buf[ch & 1][j * 256 + threadIdx.x] = r[j];
+  };
+  load(0);
+  stage(0);
+  __syncthreads();
+  double sum = 0.0;
+  for (int ch = 0; ch < nch; ++ch) {
+    if (ch + 1 < nch) load(ch + 1);
+    if (threadIdx.x == 0) {
+      const double* b = buf[ch & 1];
+      const int len = min(kMeanChunk, n - ch * kMeanChunk);
+#pragma unroll 32
+      for (int i = 0; i < len; ++i) sum += b[i];
+    }
+    if (ch + 1 < nch) stage(ch + 1);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mu[blockIdx.x] = sum / (double)n;
+}
+
+__global__ __launch_bounds__(256) void mm_first_rows_kernel(const double* __restrict__ XT, int n,
+                                                            int d, const double* __restrict__ mu,
+                                                            double* __restrict__ bval,
+                                                            int* __restrict__ bidx) {
+#pragma clang fp contract(off)
+  __shared__ double rk[256 / kWave];
+  __shared__ int ri[256 / kWave];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double key = -INFINITY;   // -(row value): the arg-max of (key, index) is the arg-min
+  int idx = INT_MAX;
+  if (i < n) {
+    double t[kMMD];
+#pragma unroll
+    for (int l = 0; l < kMMD; ++l) {
+      const double x = l < d ? XT[(int64_t)l * n + i] - mu[l] : 0.0;
+      t[l] = x * x;
+    }
+    double v;
+    if (d < 8) {
+      v = 0.0;
+#pragma unroll
+      for (int l = 0; l < 8; ++l)
+        if (l < d) v += t[l];
+    } else {
+      v = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+#pragma unroll
+      for (int l = 8; l < kMMD; ++l)
+        if (l < d) v += t[l];
+    }
+    key = -v;
+    idx = i;
+  }
+  block_argmax<256>(key, idx, rk, ri);
+  if (threadIdx.x == 0) {
+    bval[blockIdx.x] = key;
+    bidx[blockIdx.x] = idx;
+  }
+}
+
+__global__ __launch_bounds__(256) void mm_first_final_kernel(const double* __restrict__ bval,
+                                                             const int* __restrict__ bidx, int nb,
+                                                             MMDesc* __restrict__ desc) {
+  __shared__ double rk[256 / kWave];
+  __shared__ int ri[256 / kWave];
+  double key = -INFINITY;
+  int idx = INT_MAX;
+  for (int b = threadIdx.x; b < nb; b += 256)
+    if (mm_greater(bval[b], bidx[b], key, idx)) {
+      key = bval[b];
+      idx = bidx[b];
+    }
+  block_argmax<256>(key, idx, rk, ri);
+  if (threadIdx.x == 0) desc->first = idx;
+}
+
 void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMDesc>& hd,
-                          int64_t* dorder) {
+                          int64_t* dorder, bool device_first) {
   const int nd = (int)hd.size();
   if (nd == 0) return;
   int64_t tot = 0;
@@ -1582,6 +1685,20 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTopMax + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
+  if (device_first) {
+    // one ordering: its first point computed here, written into its descriptor
+    SVH_REQUIRE(nd == 1 && hd[0].soff == 0 && hd[0].xoff == 0 && hd[0].n >= 1,
+                "maximin: device first point");
+    const int n = hd[0].n, nb = (n + 255) / 256;
+    double* mu = ctx->arena.get_t<double>("mm.mu", kMMD);
+    double* bval = ctx->arena.get_t<double>("mm.fval", nb);
+    int* bidx = ctx->arena.get_t<int>("mm.fidx", nb);
+    hipLaunchKernelGGL(mm_first_mean_kernel, dim3(hd[0].d), dim3(256), 0, ctx->stream, dXT, n, mu);
+    hipLaunchKernelGGL(mm_first_rows_kernel, dim3(nb), dim3(256), 0, ctx->stream, dXT, n, hd[0].d,
+                       mu, bval, bidx);
+    hipLaunchKernelGGL(mm_first_final_kernel, dim3(1), dim3(256), 0, ctx->stream, bval, bidx, nb,
+                       dd);
+  }
   hipLaunchKernelGGL(mm_zero_kernel, dim3((nd + 255) / 256), dim3(256), 0, ctx->stream, st, nd);
   hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
                      dXT, s2, dist, acc, accx, st, dorder);
@@ -1681,11 +1798,11 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
   D.soff = 0;
   D.n = (int)n64;
   D.d = d;
-  D.first = (int)first;
+  D.first = first < 0 ? 0 : (int)first;
   D.slice0 = 0;
   D.nslice = (D.n + kSlice - 1) / kSlice;
   D.pad = 0;
-  launch_maximin_batch(ctx, dXT, std::vector<MMDesc>{D}, dorder);
+  launch_maximin_batch(ctx, dXT, std::vector<MMDesc>{D}, dorder, first < 0);
 }
 
 void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m, int64_t start,
@@ -1735,7 +1852,7 @@ void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t
   SVH_HIP(hipMemcpyAsync(dX, X, (size_t)xo * sizeof(double), hipMemcpyHostToDevice, s));
   for (int i = 0; i < nd; ++i)
     launch_to_soa(ctx, dX + hm[i].xoff, n[i], d[i], nullptr, dXT + hm[i].xoff);
-  launch_maximin_batch(ctx, dXT, hm, dord);
+  launch_maximin_batch(ctx, dXT, hm, dord, false);
   SVH_HIP(hipMemcpyAsync(order, dord, (size_t)so * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SVH_HIP(hipStreamSynchronize(s));
   // reordered inputs (host permutation keeps the exact values), then kNN
@@ -1843,7 +1960,7 @@ void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const d
                          s));
   for (int k = 0; k < count; ++k)
     launch_to_soa(ctx, dXe + lay[k].xoff, lay[k].n, lay[k].d, dlam + k * kMMD, dXT + hm[k].xoff);
-  launch_maximin_batch(ctx, dXT, hm, dord);
+  launch_maximin_batch(ctx, dXT, hm, dord, false);
   std::vector<KNNDesc> hk(count);
   for (int k = 0; k < count; ++k) {
     launch_gather_soa(ctx, dXT + hm[k].xoff, dord + hm[k].soff, lay[k].n, lay[k].d,

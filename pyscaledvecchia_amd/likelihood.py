@@ -11,7 +11,6 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib
-from .ordering import maximin_first
 
 __all__ = ["vecchia_profile_loglik", "VecchiaStructure", "DeviceRebuild"]
 
@@ -103,10 +102,11 @@ class DeviceRebuild:
         return d <= 12 and m + 1 <= 127
 
     def __call__(self, lam):
-        """Reorder at ranges `lam`; returns the maximin order."""
+        """Reorder at ranges `lam`; returns the maximin order.  The first
+        point (ordering.py:72, the row closest to the centroid of X / lam) is
+        found on the device with NumPy's arithmetic (first = -1)."""
         lam = np.ascontiguousarray(lam, dtype=np.float64)
-        Xs = self.Xe / lam
-        first = maximin_first(Xs)   # ordering.py:72
+        first = -1
         fresh = getattr(self.ctx, "_rb_owner", None) is not self
         order = self.ctx.vecchia_rebuild(self.Xe if fresh else None, self.ye, self.Ze, lam,
                                          first, self.m, upload=fresh)

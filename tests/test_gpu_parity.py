@@ -495,6 +495,30 @@ def test_device_rebuild_matches_host_structure(gpu):
         assert dev.structure(th, nu, need_grad=False)[0] == host(th, nu, need_grad=False)[0]
 
 
+@pytest.mark.parametrize("n,d", [(5000, 3), (4000, 7), (6000, 8), (3000, 10), (2500, 12),
+                                 (1, 4), (50000, 8)])
+def test_device_first_point_matches_numpy(gpu, n, d):
+    """The rebuild's first point computed on the device (first = -1) is the
+    index NumPy's argmin(((X - X.mean(0)) ** 2).sum(1)) gives for X = Xe / lam
+    (ordering.py:72), including exact ties (duplicated rows) and ranges that
+    spread the scales over decades."""
+    from pyscaledvecchia_amd.likelihood import DeviceRebuild
+    from pyscaledvecchia_amd.ordering import maximin_first
+    rng = np.random.default_rng(40 + d)
+    Xe = rng.random((n, d))
+    if n > 10:
+        Xe[n // 3] = Xe[n // 5]          # a duplicated row
+        c = Xe.mean(0)
+        Xe[7] = c
+        Xe[n - 2] = c                    # two rows at the same distance from the centroid
+    ye = rng.standard_normal(n)
+    m = min(10, n - 1)
+    dev = DeviceRebuild(Xe, ye, np.ones((n, 1)), m, ctx=gpu)
+    for lam in (np.full(d, 0.37), np.exp(rng.uniform(-3, 3, d))):
+        order = dev(lam)
+        assert order[0] == maximin_first(Xe / lam)
+
+
 @pytest.mark.parametrize("n,d,m,trend", [(2500, 4, 15, "constant"), (400, 12, 10, "zero"),
                                          (60, 3, 59, "linear")])
 def test_device_rebuild_fit_matches_reference_path(gpu, n, d, m, trend):
