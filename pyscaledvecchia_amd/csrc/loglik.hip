@@ -1300,10 +1300,55 @@ __device__ __forceinline__ void w_sweep_dpp(T (&R)[NC], double (&S)[32], double 
   wsw_hi<16>(R, S);
 }
 
+// Inputs of one block pair for lane (half h, row r): the block's size, chunk
+// and this row's point (PairIdx), then the chunk's jitter level and the row's
+// raw coordinates and response (PairX).  The rows kernel loads them ahead --
+// the indices two pairs, the values one pair before their pair -- so the
+// dependent global loads (kcnt -> nbr -> X, chunk -> level) overlap the work
+// of the pairs in between.
+struct PairIdx {
+  int K, ch, pt;
+};
+template <int DM>
+struct PairX {
+  double x[DM];
+  double y;
+  int lev;
+};
+
+__device__ __forceinline__ PairIdx pair_idx(const LLArgs& a, int64_t b0, int lane) {
+  const int h = lane >> 5, r = lane & 31;
+  const int64_t b = b0 + h;
+  PairIdx o{1, 0, 0};
+  if (b < a.nblk) {
+    const int kc = a.kcnt[b];
+    const int rw = a.rows[b];
+    const int nb = a.mmax > 0 ? a.nbr[b * a.mmax + (r < a.mmax ? r : a.mmax - 1)] : rw;
+    o.K = kc + 1;
+    o.ch = a.chunk[b];
+    o.pt = r < kc ? nb : rw;
+  }
+  return o;
+}
+
+template <int DM>
+__device__ __forceinline__ void pair_x(const LLArgs& a, int64_t b0, const PairIdx& pi, int lane,
+                                       PairX<DM>& o) {
+  const int h = lane >> 5, r = lane & 31;
+  const bool bvalid = b0 + h < a.nblk;
+  const bool act = bvalid && r < pi.K;
+  const int d = a.d;
+#pragma unroll
+  for (int l = 0; l < DM; ++l) o.x[l] = (act && l < d) ? a.X[(int64_t)pi.pt * d + l] : 0.0;
+  o.y = act ? a.y[pi.pt] : 0.0;
+  o.lev = bvalid ? a.level[pi.ch] : 0;
+}
+
 template <int NU, bool MIX, int DM, bool PROF>
 __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* lds,
                                           double (&accF)[4], double (&accX)[3],
-                                          unsigned long long (&ph)[8]) {
+                                          unsigned long long (&ph)[8], const PairIdx& pi,
+                                          const PairX<DM>& px) {
   // right-hand-side columns: [y Z] with q <= DM + 1 trend columns, and the
   // P = d + 2 <= DM + 2 derivative columns [sigma^2, ranges, nugget]
   constexpr int NC = DM + 2;
@@ -1322,12 +1367,10 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   double* misc = gt + 2 * kGTri;                      // per half: [8h] fail, [8h+1..2] pivot
   const int64_t b = b0 + h;   // first block of the pair: b0
   const bool bvalid = b < a.nblk;
-  const int K = bvalid ? a.kcnt[b] + 1 : 1;
-  const int ch = bvalid ? a.chunk[b] : 0;
-  const int lev = bvalid ? a.level[ch] : 0;
+  const int K = pi.K;
+  const int lev = px.lev;
   const bool act = bvalid && r < K;
-  int pt = 0;
-  if (act) pt = (r < K - 1) ? a.nbr[b * a.mmax + r] : a.rows[b];
+  const int pt = act ? pi.pt : 0;
   // Padding rows (r >= K) sit far apart from everything: first coordinate
   // kPadFar * (r + 1), the others 0.  Every pair with a padding point is then
   // at least kPadFar apart, so its covariance and g(r) underflow to exactly 0
@@ -1335,12 +1378,11 @@ __device__ __forceinline__ void rows_pair(const LLArgs& a, int64_t b0, double* l
   // float32 too) and need no mask.
 #pragma unroll
   for (int l = 0; l < DM; ++l) {
-    const double raw = (act && l < d) ? a.X[(int64_t)pt * d + l] : 0.0;
-    double v = (l < d) ? raw * a.inv_lam[l] : 0.0;
+    double v = (l < d) ? px.x[l] * a.inv_lam[l] : 0.0;
     if (l == 0) v = act ? v : kPadFar * (double)(r + 1);
     xs[xs_at(l, r)] = v;
   }
-  const double yv = act ? a.y[pt] : 0.0;
+  const double yv = px.y;
   const double var = a.var, nug = a.nug;
   double jit = 0.0;
   if (lev > 0) {
@@ -1711,8 +1753,19 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
   double accX[3] = {0.0, 0.0, 0.0};
   const int64_t npair = (a.nblk + 1) / 2;
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t pp = blockIdx.x; pp < npair; pp += gridDim.x)
-    rows_pair<NU, MIX, DM, PROF>(kernarg_fresh<LLArgs>(), 2 * pp, lds, accF, accX, ph);
+  const int64_t gs = gridDim.x;
+  PairIdx i0 = pair_idx(kernarg_fresh<LLArgs>(), 2 * (int64_t)blockIdx.x, lane);
+  PairIdx i1 = pair_idx(kernarg_fresh<LLArgs>(), 2 * ((int64_t)blockIdx.x + gs), lane);
+  PairX<DM> x0;
+  pair_x<DM>(kernarg_fresh<LLArgs>(), 2 * (int64_t)blockIdx.x, i0, lane, x0);
+  for (int64_t pp = blockIdx.x; pp < npair; pp += gs) {
+    const PairIdx ic = i0;
+    const PairX<DM> xc = x0;
+    i0 = i1;
+    i1 = pair_idx(kernarg_fresh<LLArgs>(), 2 * (pp + 2 * gs), lane);
+    pair_x<DM>(kernarg_fresh<LLArgs>(), 2 * (pp + gs), i0, lane, x0);
+    rows_pair<NU, MIX, DM, PROF>(kernarg_fresh<LLArgs>(), 2 * pp, lds, accF, accX, ph, ic, xc);
+  }
   if (PROF && lane == 0)
     for (int k = 0; k < 7; ++k) atomicAdd(a.prof + k, ph[k]);
   const int NACC = grad ? P * P + q * q + q : q * q + q;
@@ -2170,8 +2223,13 @@ __global__ __launch_bounds__(kWave, 2) void ll_pass1_batch_tiles(BatchArgs B,
   double accF[4] = {0.0, 0.0, 0.0, 0.0};
   double accX[3] = {0.0, 0.0, 0.0};
   unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int pp = t.p0; pp < t.p1; ++pp)
-    rows_pair<-1, false, DM, false>(a, D.blk0 + 2 * (int64_t)pp, lds, accF, accX, ph);
+  for (int pp = t.p0; pp < t.p1; ++pp) {
+    const int64_t b0 = D.blk0 + 2 * (int64_t)pp;
+    const PairIdx pi = pair_idx(a, b0, lane);
+    PairX<DM> px;
+    pair_x<DM>(a, b0, pi, lane, px);
+    rows_pair<-1, false, DM, false>(a, b0, lds, accF, accX, ph, pi, px);
+  }
   const int P = D.P, q = D.q;
   const int nX = q * q + q;
   double* part = B.partial + (int64_t)tix * kBNACC;
