@@ -3156,8 +3156,6 @@ void batch_setup_device(svh_ctx* ctx, int ndata, const int32_t* d, const int32_t
   // whose tile count is nearest 8 workgroups per CU (two waves per SIMD)
   std::vector<BTile> htiles;
   std::vector<int> hdtile((size_t)kNTileCand * 2 * ndata);
-  int64_t pairs_tot = 0;
-  for (auto& D : B.desc) pairs_tot += (D.nblk + 1) / 2;
   double best_gap = 1e300;
   for (int c = 0; c < kNTileCand; ++c) {
     const int T = kTileCand[c];

@@ -488,7 +488,7 @@ __device__ __forceinline__ void mm_conflicts_mfma(int N, const float* pf, const 
   // values selected afterwards: no branches around LDS reads
   auto coord = [&](int p, int k) {
     const float v = pf[min(p, N - 1) * DM + min(k, DM - 1)];
-    return (p < N) & (k < DM) ? v : 0.0f;
+    return ((p < N) & (k < DM)) ? v : 0.0f;
   };
 #pragma unroll 1
   for (int t = 0; t < 4; ++t) {
