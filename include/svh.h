@@ -243,7 +243,9 @@ int svh_batch_data(svh_ctx* ctx, const double* Xe, const double* ye, const doubl
 
 /* Rebuild of `count` datasets of the batch (indices which[k]) on the device:
  * the data of svh_batch_data scaled by lam[k * 12 + l] (d positive ranges),
- * maximin order from first[k], ordered m[k] nearest neighbours, and the
+ * maximin order from first[k] (first[k] = -1: the row closest to the scaled
+ * centroid, ordering.py:72, found on the device with the host's NumPy
+ * arithmetic, as for svh_vecchia_rebuild), ordered m[k] nearest neighbours, and the
  * ordered rows, neighbour lists and counts written where the fitting kernels
  * read them -- svh_batch_update without the host round trip.  Replaces the
  * rebuild of gp.py:182-191 for each dataset (d <= 10, m[k] <= the batch's

@@ -16,16 +16,14 @@ Limits of the batched kernels: d <= 10, trend columns <= 11, m_est <= 31.
 from __future__ import annotations
 
 import math
-import os
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import _lib
 from .gp import ScaledVecchiaGP
 from .likelihood import _chunk_ids, _chunk_ids_rows
-from .ordering import _nn_groups, maximin_first
+from .ordering import _nn_groups
 
 __all__ = ["fit_batch", "BatchedFit"]
 
@@ -190,22 +188,13 @@ class BatchedFit:
     @staticmethod
     def _rebuild(bh, preps, which, thetas):
         """Device rebuild of datasets `which` at `thetas` (maximin order and
-        ordered neighbours of the scaled inputs, gp.py:182-191), the first
-        point chosen on the host as ordering.py:72 does."""
+        ordered neighbours of the scaled inputs, gp.py:182-191); each first
+        point (ordering.py:72) is found on the device (first = -1) with the
+        NumPy arithmetic of the host computation."""
         lams = [np.exp(np.asarray(th, float)[1:1 + preps[i]["d"]])
                 for i, th in zip(which, thetas)]
         ms = [preps[i]["m"] for i in which]
-
-        def first(k):
-            return maximin_first(preps[which[k]]["Xe"] / lams[k])   # ordering.py:72
-        if len(which) > 4:
-            # NumPy releases the GIL in these array passes: one host thread per
-            # dataset group
-            with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-                firsts = list(ex.map(first, range(len(which))))
-        else:
-            firsts = [first(k) for k in range(len(which))]
-        bh.rebuild(which, lams, firsts, ms)
+        bh.rebuild(which, lams, [-1] * len(which), ms)
 
     @staticmethod
     def _structure(g, p, theta, mmax):

@@ -159,7 +159,8 @@ struct MMState {
 struct MMDesc {
   int64_t xoff;   // dataset's SoA coordinates start at XT + xoff (d x n)
   int64_t soff;   // s2 / dist / order start at + soff (n entries)
-  int n, d, first, slice0, nslice, pad;
+  int n, d, first, slice0, nslice;
+  int dev_first;   // nonzero: `first` is found on the device (ordering.py:72)
 };
 
 // Lane exchange with lane ^ M inside a wave, for a compile-time M: DPP quad
@@ -1573,10 +1574,14 @@ void mm_round_dm(svh_ctx* ctx, int dm, const MMRound& R) {
 // so the sequential add chain waits on LDS, not on global-memory latency.
 constexpr int kMeanChunk = 2048;   // doubles per LDS buffer (16 KB)
 
-__global__ __launch_bounds__(256) void mm_first_mean_kernel(const double* __restrict__ XT, int n,
-                                                            double* __restrict__ mu) {
+__global__ __launch_bounds__(256) void mm_first_mean_kernel(const MMDesc* __restrict__ desc,
+                                                            const double* __restrict__ XTall,
+                                                            double* __restrict__ muall) {
   __shared__ double buf[2][kMeanChunk];
-  const double* c = XT + (int64_t)blockIdx.x * n;
+  const MMDesc D = desc[blockIdx.y];
+  if (!D.dev_first || (int)blockIdx.x >= D.d) return;
+  const int n = D.n;
+  const double* c = XTall + D.xoff + (int64_t)blockIdx.x * n;
   const int nch = (n + kMeanChunk - 1) / kMeanChunk;
   double r[kMeanChunk / 256];
   auto load = [&](int ch) {
@@ -1605,16 +1610,24 @@ __global__ __launch_bounds__(256) void mm_first_mean_kernel(const double* __rest
     if (ch + 1 < nch) stage(ch + 1);
     __syncthreads();
   }
-  if (threadIdx.x == 0) mu[blockIdx.x] = sum / (double)n;
+  if (threadIdx.x == 0) muall[blockIdx.y * kMMD + blockIdx.x] = sum / (double)n;
 }
 
-__global__ __launch_bounds__(256) void mm_first_rows_kernel(const double* __restrict__ XT, int n,
-                                                            int d, const double* __restrict__ mu,
-                                                            double* __restrict__ bval,
+// Per 256-row block of each ordering: the arg-min of the rows' squared
+// distances to the centroid (as the arg-max of (-value, index)).
+__global__ __launch_bounds__(256) void mm_first_rows_kernel(const MMDesc* __restrict__ desc,
+                                                            const double* __restrict__ XTall,
+                                                            const double* __restrict__ muall,
+                                                            int nbmax, double* __restrict__ bval,
                                                             int* __restrict__ bidx) {
 #pragma clang fp contract(off)
   __shared__ double rk[256 / kWave];
   __shared__ int ri[256 / kWave];
+  const MMDesc D = desc[blockIdx.y];
+  const int n = D.n, d = D.d;
+  if (!D.dev_first || (int)blockIdx.x * 256 >= n) return;
+  const double* XT = XTall + D.xoff;
+  const double* mu = muall + blockIdx.y * kMMD;
   const int i = blockIdx.x * 256 + threadIdx.x;
   double key = -INFINITY;   // -(row value): the arg-max of (key, index) is the arg-min
   int idx = INT_MAX;
@@ -1642,16 +1655,22 @@ __global__ __launch_bounds__(256) void mm_first_rows_kernel(const double* __rest
   }
   block_argmax<256>(key, idx, rk, ri);
   if (threadIdx.x == 0) {
-    bval[blockIdx.x] = key;
-    bidx[blockIdx.x] = idx;
+    bval[(int64_t)blockIdx.y * nbmax + blockIdx.x] = key;
+    bidx[(int64_t)blockIdx.y * nbmax + blockIdx.x] = idx;
   }
 }
 
-__global__ __launch_bounds__(256) void mm_first_final_kernel(const double* __restrict__ bval,
-                                                             const int* __restrict__ bidx, int nb,
-                                                             MMDesc* __restrict__ desc) {
+__global__ __launch_bounds__(256) void mm_first_final_kernel(MMDesc* __restrict__ desc,
+                                                             const double* __restrict__ bvalall,
+                                                             const int* __restrict__ bidxall,
+                                                             int nbmax) {
   __shared__ double rk[256 / kWave];
   __shared__ int ri[256 / kWave];
+  MMDesc* D = desc + blockIdx.x;
+  if (!D->dev_first) return;
+  const int nb = (D->n + 255) / 256;
+  const double* bval = bvalall + (int64_t)blockIdx.x * nbmax;
+  const int* bidx = bidxall + (int64_t)blockIdx.x * nbmax;
   double key = -INFINITY;
   int idx = INT_MAX;
   for (int b = threadIdx.x; b < nb; b += 256)
@@ -1660,11 +1679,11 @@ __global__ __launch_bounds__(256) void mm_first_final_kernel(const double* __res
       idx = bidx[b];
     }
   block_argmax<256>(key, idx, rk, ri);
-  if (threadIdx.x == 0) desc->first = idx;
+  if (threadIdx.x == 0) D->first = idx;
 }
 
 void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMDesc>& hd,
-                          int64_t* dorder, bool device_first) {
+                          int64_t* dorder) {
   const int nd = (int)hd.size();
   if (nd == 0) return;
   int64_t tot = 0;
@@ -1685,19 +1704,24 @@ void launch_maximin_batch(svh_ctx* ctx, const double* dXT, const std::vector<MMD
   int* ci = ctx->arena.get_t<int>("mm.ci", (size_t)nslice_tot * (kSliceTopMax + 1));
   MMDesc* dd = ctx->arena.get_t<MMDesc>("mm.desc", nd);
   SVH_HIP(hipMemcpyAsync(dd, hd.data(), nd * sizeof(MMDesc), hipMemcpyHostToDevice, ctx->stream));
-  if (device_first) {
-    // one ordering: its first point computed here, written into its descriptor
-    SVH_REQUIRE(nd == 1 && hd[0].soff == 0 && hd[0].xoff == 0 && hd[0].n >= 1,
-                "maximin: device first point");
-    const int n = hd[0].n, nb = (n + 255) / 256;
-    double* mu = ctx->arena.get_t<double>("mm.mu", kMMD);
-    double* bval = ctx->arena.get_t<double>("mm.fval", nb);
-    int* bidx = ctx->arena.get_t<int>("mm.fidx", nb);
-    hipLaunchKernelGGL(mm_first_mean_kernel, dim3(hd[0].d), dim3(256), 0, ctx->stream, dXT, n, mu);
-    hipLaunchKernelGGL(mm_first_rows_kernel, dim3(nb), dim3(256), 0, ctx->stream, dXT, n, hd[0].d,
-                       mu, bval, bidx);
-    hipLaunchKernelGGL(mm_first_final_kernel, dim3(1), dim3(256), 0, ctx->stream, bval, bidx, nb,
-                       dd);
+  int nbmax = 0, dfirst = 0;
+  for (const auto& D : hd)
+    if (D.dev_first) {
+      SVH_REQUIRE(D.n >= 1, "maximin: device first point of an empty input");
+      nbmax = std::max(nbmax, (D.n + 255) / 256);
+      ++dfirst;
+    }
+  if (dfirst > 0) {
+    // first points found here (ordering.py:72) and written into the descriptors
+    double* mu = ctx->arena.get_t<double>("mm.mu", (size_t)nd * kMMD);
+    double* bval = ctx->arena.get_t<double>("mm.fval", (size_t)nd * nbmax);
+    int* bidx = ctx->arena.get_t<int>("mm.fidx", (size_t)nd * nbmax);
+    hipLaunchKernelGGL(mm_first_mean_kernel, dim3(dmax, nd), dim3(256), 0, ctx->stream, dd, dXT,
+                       mu);
+    hipLaunchKernelGGL(mm_first_rows_kernel, dim3(nbmax, nd), dim3(256), 0, ctx->stream, dd, dXT,
+                       mu, nbmax, bval, bidx);
+    hipLaunchKernelGGL(mm_first_final_kernel, dim3(nd), dim3(256), 0, ctx->stream, dd, bval, bidx,
+                       nbmax);
   }
   hipLaunchKernelGGL(mm_zero_kernel, dim3((nd + 255) / 256), dim3(256), 0, ctx->stream, st, nd);
   hipLaunchKernelGGL(mm_init_kernel, dim3((maxn + 255) / 256, nd), dim3(256), 0, ctx->stream, dd,
@@ -1801,8 +1825,8 @@ void launch_maximin(svh_ctx* ctx, const double* dXT, int64_t n64, int d, int64_t
   D.first = first < 0 ? 0 : (int)first;
   D.slice0 = 0;
   D.nslice = (D.n + kSlice - 1) / kSlice;
-  D.pad = 0;
-  launch_maximin_batch(ctx, dXT, std::vector<MMDesc>{D}, dorder, first < 0);
+  D.dev_first = first < 0 ? 1 : 0;
+  launch_maximin_batch(ctx, dXT, std::vector<MMDesc>{D}, dorder);
 }
 
 void launch_ordered_knn(svh_ctx* ctx, const double* dXT, int64_t n, int d, int m, int64_t start,
@@ -1838,7 +1862,7 @@ void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t
     D.first = (int)first[i];
     D.slice0 = sl;
     D.nslice = (D.n + kSlice - 1) / kSlice;
-    D.pad = 0;
+    D.dev_first = 0;
     xo += n[i] * d[i];
     so += n[i];
     sl += D.nslice;
@@ -1852,7 +1876,7 @@ void ordering_batch_device(svh_ctx* ctx, int nd, const int64_t* n, const int32_t
   SVH_HIP(hipMemcpyAsync(dX, X, (size_t)xo * sizeof(double), hipMemcpyHostToDevice, s));
   for (int i = 0; i < nd; ++i)
     launch_to_soa(ctx, dX + hm[i].xoff, n[i], d[i], nullptr, dXT + hm[i].xoff);
-  launch_maximin_batch(ctx, dXT, hm, dord, false);
+  launch_maximin_batch(ctx, dXT, hm, dord);
   SVH_HIP(hipMemcpyAsync(order, dord, (size_t)so * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   SVH_HIP(hipStreamSynchronize(s));
   // reordered inputs (host permutation keeps the exact values), then kNN
@@ -1927,7 +1951,7 @@ void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const d
     const BatchLayout L = batch_layout(ctx, which[k]);
     SVH_REQUIRE(L.d <= kMMD, "batch rebuild: d above 12");
     SVH_REQUIRE(m[k] >= 0 && m[k] <= L.mmax && m[k] < L.n, "batch rebuild: bad m");
-    SVH_REQUIRE(first[k] >= 0 && first[k] < L.n, "batch rebuild: first out of range");
+    SVH_REQUIRE(first[k] >= -1 && first[k] < L.n, "batch rebuild: first out of range");
     for (int l = 0; l < L.d; ++l)
       SVH_REQUIRE(std::isfinite(lam[k * kMMD + l]) && lam[k * kMMD + l] > 0.0,
                   "batch rebuild: ranges must be positive");
@@ -1937,10 +1961,10 @@ void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const d
     D.soff = so;
     D.n = (int)L.n;
     D.d = L.d;
-    D.first = (int)first[k];
+    D.first = first[k] < 0 ? 0 : (int)first[k];
     D.slice0 = sl;
     D.nslice = (D.n + kSlice - 1) / kSlice;
-    D.pad = 0;
+    D.dev_first = first[k] < 0 ? 1 : 0;
     xo += L.n * L.d;
     so += L.n;
     sl += D.nslice;
@@ -1960,7 +1984,7 @@ void batch_rebuild_device(svh_ctx* ctx, int count, const int32_t* which, const d
                          s));
   for (int k = 0; k < count; ++k)
     launch_to_soa(ctx, dXe + lay[k].xoff, lay[k].n, lay[k].d, dlam + k * kMMD, dXT + hm[k].xoff);
-  launch_maximin_batch(ctx, dXT, hm, dord, false);
+  launch_maximin_batch(ctx, dXT, hm, dord);
   std::vector<KNNDesc> hk(count);
   for (int k = 0; k < count; ++k) {
     launch_gather_soa(ctx, dXT + hm[k].xoff, dord + hm[k].soff, lay[k].n, lay[k].d,

@@ -169,3 +169,49 @@ def test_batch_setup_rejects_null_trend_with_columns(gpu):
                            ip(fixed), dp(lnug), 1e3)
     assert rc == 2
     assert b"Z is NULL" in L.svh_last_error(gpu._h)
+
+
+def test_batch_rebuild_device_first_point(gpu):
+    """Batched rebuilds with first = -1 (each dataset's first point found on
+    the device) give the same structures as with the first points NumPy
+    computes on the host (ordering.py:72): the objective of every dataset is
+    bit-identical.  One dataset holds two rows at the same distance from the
+    centroid (the lower index wins), one a duplicated row."""
+    from pyscaledvecchia_amd import _lib
+    from pyscaledvecchia_amd.gp import ScaledVecchiaGP
+    from pyscaledvecchia_amd.likelihood import _chunk_ids_rows
+    from pyscaledvecchia_amd.ordering import maximin_first
+    data = _datasets(11, nd=4)
+    X0, y0 = data[0]
+    c = X0.mean(0)
+    X0[5] = c
+    X0[len(X0) - 3] = c
+    X1, y1 = data[1]
+    X1[len(X1) // 2] = X1[3]
+    models = [ScaledVecchiaGP(m_est=15, nu=2.5, trend="linear", n_est=2000) for _ in data]
+    preps = [g._prepare(X, y) for g, (X, y) in zip(models, data)]
+    nd = len(data)
+    mmax = max(p["m"] for p in preps)
+    chunks = [_chunk_ids_rows(len(p["ye"]), p["m"], p["d"] + 2) for p in preps]
+    bh = _lib.BatchHandle(gpu)
+    bh.setup([p["d"] for p in preps], [p["Ze"].shape[1] for p in preps], [2] * nd,
+             [len(p["ye"]) for p in preps], None, None, None, None, None,
+             np.concatenate([c for c, _ in chunks]), [nc for _, nc in chunks], mmax,
+             [p["lvt"] for p in preps], [0] * nd, [0.0] * nd, models[0].lambda_max)
+    bh.data(np.concatenate([p["Xe"].ravel() for p in preps]),
+            np.concatenate([p["ye"] for p in preps]),
+            np.concatenate([p["Ze"].ravel() for p in preps]))
+    rng = np.random.default_rng(4)
+    th = np.zeros((nd, 12))
+    for i, p in enumerate(preps):
+        th[i, :p["d"] + 2] = p["theta"] + 0.2 * rng.standard_normal(p["d"] + 2)
+    lams = [np.exp(th[i, 1:1 + p["d"]]) for i, p in enumerate(preps)]
+    which = list(range(nd))
+    host = [maximin_first(p["Xe"] / la) for p, la in zip(preps, lams)]
+    ms = [p["m"] for p in preps]
+    bh.rebuild(which, lams, host, ms)
+    r_host = bh.eval(th)
+    bh.rebuild(which, lams, [-1] * nd, ms)
+    r_dev = bh.eval(th)
+    for a, b in zip(r_host, r_dev):
+        np.testing.assert_array_equal(np.asarray(a), np.asarray(b))
