@@ -180,7 +180,6 @@ def test_batch_rebuild_device_first_point(gpu):
     from pyscaledvecchia_amd import _lib
     from pyscaledvecchia_amd.gp import ScaledVecchiaGP
     from pyscaledvecchia_amd.likelihood import _chunk_ids_rows
-    from pyscaledvecchia_amd.ordering import maximin_first
     data = _datasets(11, nd=4)
     X0, y0 = data[0]
     c = X0.mean(0)
@@ -207,7 +206,7 @@ def test_batch_rebuild_device_first_point(gpu):
         th[i, :p["d"] + 2] = p["theta"] + 0.2 * rng.standard_normal(p["d"] + 2)
     lams = [np.exp(th[i, 1:1 + p["d"]]) for i, p in enumerate(preps)]
     which = list(range(nd))
-    host = [maximin_first(p["Xe"] / la) for p, la in zip(preps, lams)]
+    host = [O.maximin_first(p["Xe"] / la) for p, la in zip(preps, lams)]
     ms = [p["m"] for p in preps]
     bh.rebuild(which, lams, host, ms)
     r_host = bh.eval(th)
