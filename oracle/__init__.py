@@ -21,6 +21,8 @@ PINS = {
                               "central finite difference of M for g=M'/r",
     "maximin_order": "pure-Python brute force of the definition (tiny n); "
                      "permutation + determinism invariants (reference tests)",
+    "maximin_first": "known answers: middle point of an evenly spaced line (lower index "
+                     "of the two middle ones for even n), a row placed at the centroid",
     "find_ordered_nn": "pure-Python brute force per row (reference test_ordering); "
                        "structure: self column, -1 padding, start= invariance",
     "block_cov": "central finite differences of Sigma(theta) in log-parameters; "

@@ -81,6 +81,25 @@ def test_maximin_ties_take_lowest_index():
     np.testing.assert_array_equal(O.maximin_order(X), _maximin_bruteforce(X, first))
 
 
+def test_maximin_first_known_answers():
+    """The first point (ordering.py:72) where the answer is fixed by geometry:
+    equally spaced points on a line (odd n: the middle one; even n: the two
+    middle ones tie and the lower index wins), a point placed at the centroid
+    of random data, and the centroid shared by two rows (the lower index)."""
+    for n in (7, 101):
+        X = np.arange(n, dtype=float)[:, None] * np.array([[1.0, -2.0, 0.5]])
+        assert O.maximin_first(X) == n // 2
+    X = np.arange(10, dtype=float)[:, None] * np.array([[1.0, 3.0]])
+    assert O.maximin_first(X) == 4
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((500, 4))
+    X = X - X.mean(0)                  # centroid at 0 up to rounding
+    X[123] = 0.0
+    assert O.maximin_first(X) == 123
+    X[40] = 0.0
+    assert O.maximin_first(X) == 40
+
+
 def test_maximin_permutation_and_deterministic():
     X = np.random.default_rng(0).random((200, 4))
     o = O.maximin_order(X)
