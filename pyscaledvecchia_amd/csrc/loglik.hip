@@ -1776,7 +1776,7 @@ __global__ __launch_bounds__(kWave, OCC) void ll_pass1_rows(LLArgs a) {
     part[X0 + e] = (e < kWave) ? accX[0] : (e < 2 * kWave ? accX[1] : accX[2]);
 }
 
-// two waves per SIMD (at most 256 registers; the kernel needs about 211)
+// two waves per SIMD (at most 256 registers; the kernel needs about 219)
 template <int NU, int DM>
 void launch_rows_nu_dm(hipStream_t st, int G, size_t lds, const LLArgs& a, bool mixed) {
   if constexpr (NU == 2 && DM == 8) {   // phase clocks: FP64, nu = 5/2, d in 5..8
@@ -2850,7 +2850,7 @@ void vecchia_eval_device(svh_ctx* ctx, const double* theta, int P, double nu, in
   int dflt = 0;
   const int64_t npair = (vs.nblk + 1) / 2;
   if (use_rows) {
-    // kind 2: rows kernel at 2 waves per SIMD (211 registers, no spills; the
+    // kind 2: rows kernel at 2 waves per SIMD (219 registers, no spills; the
     // 3-wave build spills about 60 registers, and its scratch traffic, up to
     // 0.7 GB per launch at C2, buys no measurable speed: within 2 %),
     // default 16 workgroups per CU
